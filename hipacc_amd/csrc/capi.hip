@@ -1,0 +1,582 @@
+// capi.hip -- the extern "C" entry points of libhipacc_gfx950.so: argument
+// validation, descriptor -> kernel-argument translation, per-device state,
+// timing and error reporting.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hh_tile.hpp"
+
+namespace hh {
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+static thread_local std::string t_last_error;
+
+int set_error(int status, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  t_last_error = buf;
+  return status;
+}
+
+int hip_status(hipError_t e, const char* what) {
+  if (e == hipSuccess) return HIPACC_OK;
+  return set_error(e == hipErrorOutOfMemory ? HIPACC_ERR_OUT_OF_MEMORY : HIPACC_ERR_HIP,
+                   "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+}
+
+// ---------------------------------------------------------------------------
+// Per-device state: the reduction workspace.
+// ---------------------------------------------------------------------------
+constexpr int kMaxDevices = 64;
+constexpr size_t kRedWorkspace = 1024 * sizeof(unsigned long long) + 256;
+
+struct DeviceState {
+  std::mutex mu;
+  bool ready = false;
+  void* red_ws = nullptr;
+};
+static DeviceState g_dev[kMaxDevices];
+
+static int current_device(int* dev) {
+  hipError_t e = hipGetDevice(dev);
+  if (e != hipSuccess) return hip_status(e, "hipGetDevice");
+  if (*dev < 0 || *dev >= kMaxDevices) return set_error(HIPACC_ERR_NO_DEVICE, "device %d", *dev);
+  return HIPACC_OK;
+}
+
+static int ensure_workspace(int dev, void** ws) {
+  DeviceState& d = g_dev[dev];
+  std::lock_guard<std::mutex> lk(d.mu);
+  if (!d.red_ws) {
+    hipError_t e = hipMalloc(&d.red_ws, kRedWorkspace);
+    if (e != hipSuccess) return hip_status(e, "hipMalloc(reduction workspace)");
+    e = hipMemset(d.red_ws, 0, kRedWorkspace);
+    if (e != hipSuccess) return hip_status(e, "hipMemset(reduction workspace)");
+    e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_status(e, "hipDeviceSynchronize");
+  }
+  *ws = d.red_ws;
+  return HIPACC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Timing (hipacc_last_kernel_timing)
+// ---------------------------------------------------------------------------
+static bool g_timing = false;
+static float g_last_ms = 0.0f;
+
+struct Timer {
+  hipStream_t s;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  explicit Timer(hipStream_t st) : s(st) {
+    if (g_timing && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess)
+      hipEventRecord(e0, s);
+  }
+  int finish(int st) {
+    if (e0 && e1) {
+      if (st == HIPACC_OK) {
+        hipEventRecord(e1, s);
+        hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) g_last_ms = ms;
+      }
+      hipEventDestroy(e0);
+      hipEventDestroy(e1);
+      e0 = e1 = nullptr;
+    }
+    return st;
+  }
+  ~Timer() {
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Descriptor checks
+// ---------------------------------------------------------------------------
+static bool valid_dtype(int dt) { return dt >= HIPACC_U8 && dt <= HIPACC_F32; }
+
+static int check_view(const hipacc_view& v, const char* name) {
+  if (!v.data) return set_error(HIPACC_ERR_INVALID_ARG, "%s: data is NULL", name);
+  if (!valid_dtype(v.dtype)) return set_error(HIPACC_ERR_INVALID_ARG, "%s: dtype %d", name, v.dtype);
+  if (v.img_width <= 0 || v.img_height <= 0 || v.stride < v.img_width)
+    return set_error(HIPACC_ERR_INVALID_ARG, "%s: image %dx%d stride %d", name, v.img_width,
+                     v.img_height, v.stride);
+  if (v.width <= 0 || v.height <= 0 || v.offset_x < 0 || v.offset_y < 0 ||
+      v.offset_x + v.width > v.img_width || v.offset_y + v.height > v.img_height)
+    return set_error(HIPACC_ERR_INVALID_ARG, "%s: window %dx%d+%d+%d outside image %dx%d", name,
+                     v.width, v.height, v.offset_x, v.offset_y, v.img_width, v.img_height);
+  if (v.batch < 0) return set_error(HIPACC_ERR_INVALID_ARG, "%s: batch %d", name, v.batch);
+  if (v.batch > 1 && v.plane_stride < (int64_t)v.stride * v.img_height)
+    return set_error(HIPACC_ERR_INVALID_ARG, "%s: plane_stride %lld too small", name,
+                     (long long)v.plane_stride);
+  return HIPACC_OK;
+}
+
+static int batch_of(const hipacc_view& v) { return v.batch > 1 ? v.batch : 1; }
+
+// Whether every row start of the window is aligned to `align` bytes.
+static bool rows_aligned(const hipacc_view& v, int align) {
+  const int eb = dtype_bytes(v.dtype);
+  const uintptr_t p = reinterpret_cast<uintptr_t>(v.data);
+  if (p % align) return false;
+  if (((long long)v.stride * eb) % align) return false;
+  if (((long long)v.offset_x * eb) % align) return false;
+  if (batch_of(v) > 1 && (v.plane_stride * eb) % align) return false;
+  return true;
+}
+
+static void const_views(int dtype, double cv, float* cf, int* ci) {
+  switch (dtype) {
+    case HIPACC_U8: *ci = (uint8_t)(long long)cv; break;
+    case HIPACC_I8: *ci = (int8_t)(long long)cv; break;
+    case HIPACC_U16: *ci = (uint16_t)(long long)cv; break;
+    case HIPACC_I16: *ci = (int16_t)(long long)cv; break;
+    case HIPACC_U32: *ci = (int32_t)(uint32_t)(long long)cv; break;
+    case HIPACC_I32: *ci = (int32_t)(long long)cv; break;
+    default:
+      *cf = (float)cv;
+      *ci = (int32_t)*cf;
+      return;
+  }
+  *cf = dtype == HIPACC_U32 ? (float)(uint32_t)*ci : (float)*ci;
+}
+
+static int make_geom(Geom& g, const hipacc_view& in, const hipacc_view& out, int boundary,
+                     double const_val, int cvt, float post_div) {
+  int st = check_view(in, "in");
+  if (st) return st;
+  if ((st = check_view(out, "out"))) return st;
+  if (batch_of(in) != batch_of(out))
+    return set_error(HIPACC_ERR_INVALID_ARG, "batch of in (%d) and out (%d) differ", batch_of(in),
+                     batch_of(out));
+  if (boundary < HIPACC_BOUNDARY_UNDEFINED || boundary > HIPACC_BOUNDARY_CONSTANT)
+    return set_error(HIPACC_ERR_INVALID_ARG, "boundary %d", boundary);
+  if (cvt != HIPACC_CVT_TRUNC && cvt != HIPACC_CVT_ROUND)
+    return set_error(HIPACC_ERR_INVALID_ARG, "cvt %d", cvt);
+  std::memset(&g, 0, sizeof(g));
+  g.in = in.data;
+  g.out = out.data;
+  g.in_plane = in.plane_stride;
+  g.out_plane = out.plane_stride;
+  g.in_stride = in.stride;
+  g.out_stride = out.stride;
+  g.in_dtype = in.dtype;
+  g.out_dtype = out.dtype;
+  g.in_imgw = in.img_width;
+  g.in_imgh = in.img_height;
+  g.lo_x = in.offset_x;
+  g.lo_y = in.offset_y;
+  g.hi_x = in.offset_x + in.width;
+  g.hi_y = in.offset_y + in.height;
+  g.in_ox = in.offset_x;
+  g.in_oy = in.offset_y;
+  g.out_ox = out.offset_x;
+  g.out_oy = out.offset_y;
+  g.is_w = out.width;
+  g.is_h = out.height;
+  g.batch = batch_of(in);
+  g.boundary = boundary;
+  const_views(in.dtype, const_val, &g.const_f, &g.const_i);
+  g.cvt = cvt;
+  g.post_div = post_div;
+  return HIPACC_OK;
+}
+
+}  // namespace hh
+
+using namespace hh;
+
+extern "C" {
+
+const char* hipacc_hip_version(void) { return "hipacc-gfx950 0.1"; }
+
+const char* hipacc_hip_strerror(int status) {
+  switch (status) {
+    case HIPACC_OK: return "ok";
+    case HIPACC_ERR_INVALID_ARG: return "invalid argument";
+    case HIPACC_ERR_UNSUPPORTED: return "unsupported";
+    case HIPACC_ERR_HIP: return "HIP runtime error";
+    case HIPACC_ERR_NO_DEVICE: return "no usable device";
+    case HIPACC_ERR_OUT_OF_MEMORY: return "out of device memory";
+  }
+  return "unknown status";
+}
+
+const char* hipacc_hip_last_error(void) { return t_last_error.c_str(); }
+
+int hipacc_hip_init(int device) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return set_error(HIPACC_ERR_NO_DEVICE, "no HIP device");
+  if (device < 0 || device >= n || device >= kMaxDevices)
+    return set_error(HIPACC_ERR_NO_DEVICE, "device %d of %d", device, n);
+  if ((e = hipSetDevice(device)) != hipSuccess) return hip_status(e, "hipSetDevice");
+  hipDeviceProp_t prop;
+  if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess)
+    return hip_status(e, "hipGetDeviceProperties");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return set_error(HIPACC_ERR_NO_DEVICE, "device %d is %s, this library is built for gfx950",
+                     device, prop.gcnArchName);
+  void* ws = nullptr;
+  int st = ensure_workspace(device, &ws);
+  if (st) return st;
+  g_dev[device].ready = true;
+  return HIPACC_OK;
+}
+
+int hipacc_hip_set_timing(int enabled) {
+  g_timing = enabled != 0;
+  return HIPACC_OK;
+}
+float hipacc_hip_last_kernel_timing(void) { return g_last_ms; }
+
+int hipacc_hip_local(const hipacc_local_desc* d, void* stream) {
+  if (!d) return set_error(HIPACC_ERR_INVALID_ARG, "local: desc is NULL");
+  LocalArgs a;
+  std::memset(&a, 0, sizeof(a));
+  int st = make_geom(a.g, d->in, d->out, d->boundary, d->const_val, d->cvt, d->post_div);
+  if (st) return st;
+  if (d->mask_w < 1 || d->mask_h < 1 || d->mask_w > HIPACC_MAX_MASK_DIM ||
+      d->mask_h > HIPACC_MAX_MASK_DIM || !(d->mask_w & 1) || !(d->mask_h & 1))
+    return set_error(HIPACC_ERR_INVALID_ARG, "local: mask %dx%d (odd, <= %d)", d->mask_w,
+                     d->mask_h, HIPACC_MAX_MASK_DIM);
+  if (d->acc_dtype != HIPACC_F32 && d->acc_dtype != HIPACC_I32)
+    return set_error(HIPACC_ERR_INVALID_ARG, "local: acc_dtype must be F32 or I32");
+  if (d->reduce < HIPACC_REDUCE_SUM || d->reduce > HIPACC_REDUCE_PROD)
+    return set_error(HIPACC_ERR_INVALID_ARG, "local: reduce %d", d->reduce);
+  const int n = d->mask_w * d->mask_h;
+  const bool accf = d->acc_dtype == HIPACC_F32;
+  if (d->use_weights) {
+    if (d->weights_dev) {
+      if (!accf) return set_error(HIPACC_ERR_INVALID_ARG, "local: weights_dev needs acc_dtype F32");
+    } else if (accf) {
+      if (d->weights_f32) {
+        std::memcpy(a.w.f, d->weights_f32, n * sizeof(float));
+      } else if (d->weights_i32) {
+        for (int t = 0; t < n; ++t) a.w.f[t] = (float)d->weights_i32[t];
+      } else {
+        return set_error(HIPACC_ERR_INVALID_ARG, "local: use_weights without weights");
+      }
+    } else {
+      if (d->weights_i32) {
+        std::memcpy(a.w.i, d->weights_i32, n * sizeof(int32_t));
+      } else if (d->weights_f32) {
+        for (int t = 0; t < n; ++t) a.w.i[t] = (int32_t)d->weights_f32[t];
+      } else {
+        return set_error(HIPACC_ERR_INVALID_ARG, "local: use_weights without weights");
+      }
+    }
+  }
+  bool any = false;
+  for (int t = 0; t < n; ++t) {
+    a.domain[t] = d->domain ? (d->domain[t] != 0) : 1;
+    any = any || a.domain[t];
+  }
+  if (!any) return set_error(HIPACC_ERR_INVALID_ARG, "local: empty domain");
+  a.mask_w = d->mask_w;
+  a.mask_h = d->mask_h;
+  a.reduce = d->reduce;
+  a.use_weights = d->use_weights != 0;
+  a.fma = d->fp_contract != 0;
+  a.wdev = d->use_weights ? d->weights_dev : nullptr;
+  const int ebi = dtype_bytes(d->in.dtype), ebo = dtype_bytes(d->out.dtype);
+  const int vec = 16 / ebi;
+  a.in_vec_ok = rows_aligned(d->in, 16) && (d->in.offset_x % vec) == 0;
+  a.out_vec_ok = rows_aligned(d->out, std::min(16, vec * ebo));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Timer tm(s);
+  return tm.finish(launch_local(a, accf, s, 0));
+}
+
+int hipacc_hip_separable(const hipacc_separable_desc* d, void* stream) {
+  if (!d) return set_error(HIPACC_ERR_INVALID_ARG, "separable: desc is NULL");
+  SepArgs a;
+  std::memset(&a, 0, sizeof(a));
+  int st = make_geom(a.g, d->in, d->out, d->boundary, d->const_val, d->cvt, d->post_div);
+  if (st) return st;
+  const int kmax = 2 * HIPACC_MAX_SEP_RADIUS + 1;
+  if (d->ksize_x < 1 || d->ksize_y < 1 || d->ksize_x > kmax || d->ksize_y > kmax ||
+      !(d->ksize_x & 1) || !(d->ksize_y & 1))
+    return set_error(HIPACC_ERR_INVALID_ARG, "separable: ksize %dx%d", d->ksize_x, d->ksize_y);
+  if (d->in.width != d->out.width || d->in.height != d->out.height)
+    return set_error(HIPACC_ERR_UNSUPPORTED,
+                     "separable: iteration space must have the accessor window's size");
+  const bool accf = d->acc_dtype == HIPACC_F32;
+  if (!accf && d->acc_dtype != HIPACC_I32)
+    return set_error(HIPACC_ERR_INVALID_ARG, "separable: acc_dtype must be F32 or I32");
+  a.kx = d->ksize_x;
+  a.ky = d->ksize_y;
+  a.box = 0;
+  if (accf) {
+    if (!d->row_f32 || !d->col_f32) return set_error(HIPACC_ERR_INVALID_ARG, "separable: float weights missing");
+    std::memcpy(a.rowf, d->row_f32, a.kx * sizeof(float));
+    std::memcpy(a.colf, d->col_f32, a.ky * sizeof(float));
+  } else {
+    if (!d->row_i32 || !d->col_i32) return set_error(HIPACC_ERR_INVALID_ARG, "separable: integer weights missing");
+    std::memcpy(a.rowi, d->row_i32, a.kx * sizeof(int32_t));
+    std::memcpy(a.coli, d->col_i32, a.ky * sizeof(int32_t));
+    bool ones = true;
+    for (int i = 0; i < a.kx; ++i) ones = ones && a.rowi[i] == 1;
+    for (int i = 0; i < a.ky; ++i) ones = ones && a.coli[i] == 1;
+    a.box = ones;
+  }
+  const int ebi = dtype_bytes(d->in.dtype), ebo = dtype_bytes(d->out.dtype);
+  const int vec = 16 / ebi;
+  a.in_vec_ok = rows_aligned(d->in, 16) && (d->in.offset_x % vec) == 0;
+  a.out_vec_ok = rows_aligned(d->out, std::min(16, vec * ebo));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Timer tm(s);
+  return tm.finish(launch_separable(a, accf, d->fp_contract != 0, s, 0));
+}
+
+int hipacc_hip_point(const hipacc_point_desc* d, void* stream) {
+  if (!d) return set_error(HIPACC_ERR_INVALID_ARG, "point: desc is NULL");
+  int st = check_view(d->in0, "in0");
+  if (st) return st;
+  if ((st = check_view(d->out, "out"))) return st;
+  const bool binary = d->op >= HIPACC_POINT_ADD && d->op <= HIPACC_POINT_MAGNITUDE;
+  if (d->op < HIPACC_POINT_COPY || d->op > HIPACC_POINT_MAGNITUDE)
+    return set_error(HIPACC_ERR_INVALID_ARG, "point: op %d", d->op);
+  if (binary && (st = check_view(d->in1, "in1"))) return st;
+  const hipacc_view& o = d->out;
+  if (d->in0.width < o.width || d->in0.height < o.height ||
+      (binary && (d->in1.width < o.width || d->in1.height < o.height)))
+    return set_error(HIPACC_ERR_INVALID_ARG, "point: input window smaller than iteration space");
+  if (batch_of(d->in0) != batch_of(o) || (binary && batch_of(d->in1) != batch_of(o)))
+    return set_error(HIPACC_ERR_INVALID_ARG, "point: batch mismatch");
+  if (d->cvt != HIPACC_CVT_TRUNC && d->cvt != HIPACC_CVT_ROUND)
+    return set_error(HIPACC_ERR_INVALID_ARG, "point: cvt %d", d->cvt);
+  PointArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.in0 = d->in0.data;
+  a.in1 = binary ? d->in1.data : nullptr;
+  a.out = o.data;
+  a.plane0 = d->in0.plane_stride;
+  a.plane1 = binary ? d->in1.plane_stride : 0;
+  a.plane_out = o.plane_stride;
+  a.stride0 = d->in0.stride;
+  a.stride1 = binary ? d->in1.stride : 0;
+  a.stride_out = o.stride;
+  a.ox0 = d->in0.offset_x;
+  a.oy0 = d->in0.offset_y;
+  a.ox1 = binary ? d->in1.offset_x : 0;
+  a.oy1 = binary ? d->in1.offset_y : 0;
+  a.oxo = o.offset_x;
+  a.oyo = o.offset_y;
+  a.w = o.width;
+  a.h = o.height;
+  a.batch = batch_of(o);
+  a.op = d->op;
+  a.cvt = d->cvt;
+  a.alpha = d->alpha;
+  a.beta = d->beta;
+  const int e0 = dtype_bytes(d->in0.dtype), eo = dtype_bytes(o.dtype);
+  a.vec_ok = rows_aligned(d->in0, 4 * e0) && rows_aligned(o, 4 * eo) &&
+             (!binary || rows_aligned(d->in1, 4 * dtype_bytes(d->in1.dtype)));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Timer tm(s);
+  return tm.finish(launch_point(a, d->in0.dtype, binary ? d->in1.dtype : d->in0.dtype, o.dtype, s));
+}
+
+size_t hipacc_hip_reduce_workspace_bytes(void) { return kRedWorkspace; }
+
+int hipacc_hip_reduce(const hipacc_view* img, int32_t reduce, int32_t acc_dtype, void* result_dev,
+                      void* workspace, void* stream) {
+  if (!img || !result_dev) return set_error(HIPACC_ERR_INVALID_ARG, "reduce: NULL argument");
+  int st = check_view(*img, "img");
+  if (st) return st;
+  if (batch_of(*img) != 1) return set_error(HIPACC_ERR_UNSUPPORTED, "reduce: batch > 1");
+  if (reduce < HIPACC_REDUCE_SUM || reduce > HIPACC_REDUCE_PROD)
+    return set_error(HIPACC_ERR_INVALID_ARG, "reduce: mode %d", reduce);
+  if (acc_dtype != HIPACC_F32 && acc_dtype != HIPACC_U32 && acc_dtype != HIPACC_I32)
+    return set_error(HIPACC_ERR_INVALID_ARG, "reduce: acc_dtype %d", acc_dtype);
+  if ((img->dtype == HIPACC_F32) != (acc_dtype == HIPACC_F32))
+    return set_error(HIPACC_ERR_UNSUPPORTED, "reduce: float images need a float accumulator");
+  if (!workspace) {
+    int dev;
+    if ((st = current_device(&dev))) return st;
+    if ((st = ensure_workspace(dev, &workspace))) return st;
+  }
+  ReduceArgs a;
+  a.in = img->data;
+  a.stride = img->stride;
+  a.ox = img->offset_x;
+  a.oy = img->offset_y;
+  a.w = img->width;
+  a.h = img->height;
+  a.mode = reduce;
+  a.vec_ok = rows_aligned(*img, 16);
+  a.result = result_dev;
+  a.partials = static_cast<unsigned long long*>(workspace);
+  a.ticket = reinterpret_cast<unsigned int*>(static_cast<char*>(workspace) + 1024 * sizeof(unsigned long long));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Timer tm(s);
+  return tm.finish(launch_reduce(a, img->dtype, acc_dtype, s));
+}
+
+int hipacc_hip_reduce_sync(const hipacc_view* img, int32_t reduce, int32_t acc_dtype,
+                           void* result_host, void* stream) {
+  if (!result_host) return set_error(HIPACC_ERR_INVALID_ARG, "reduce_sync: result_host is NULL");
+  int dev, st;
+  if ((st = current_device(&dev))) return st;
+  void* ws = nullptr;
+  if ((st = ensure_workspace(dev, &ws))) return st;
+  // The scalar result lives at the end of the library workspace.
+  void* res = static_cast<char*>(ws) + 1024 * sizeof(unsigned long long) + 64;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if ((st = hipacc_hip_reduce(img, reduce, acc_dtype, res, ws, stream))) return st;
+  hipError_t e = hipMemcpyAsync(result_host, res, 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return hip_status(e, "reduce_sync copy-back");
+}
+
+int hipacc_hip_histogram(const hipacc_view* img, int32_t num_bins, float scale, uint32_t* bins_dev,
+                         void* stream) {
+  if (!img || !bins_dev) return set_error(HIPACC_ERR_INVALID_ARG, "histogram: NULL argument");
+  int st = check_view(*img, "img");
+  if (st) return st;
+  if (batch_of(*img) != 1) return set_error(HIPACC_ERR_UNSUPPORTED, "histogram: batch > 1");
+  if (num_bins < 1 || num_bins > HIPACC_MAX_BINS)
+    return set_error(HIPACC_ERR_INVALID_ARG, "histogram: num_bins %d", num_bins);
+  HistArgs a;
+  a.in = img->data;
+  a.stride = img->stride;
+  a.ox = img->offset_x;
+  a.oy = img->offset_y;
+  a.w = img->width;
+  a.h = img->height;
+  a.nbins = num_bins;
+  a.scale = scale;
+  a.vec_ok = rows_aligned(*img, 16);
+  a.bins = bins_dev;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Timer tm(s);
+  st = launch_fill(bins_dev, 0u, (size_t)num_bins, 4, s);
+  if (st) return tm.finish(st);
+  return tm.finish(launch_histogram(a, img->dtype, s));
+}
+
+int hipacc_hip_bilateral(const hipacc_bilateral_desc* d, void* stream) {
+  if (!d) return set_error(HIPACC_ERR_INVALID_ARG, "bilateral: desc is NULL");
+  BilatArgs a;
+  std::memset(&a, 0, sizeof(a));
+  int st = make_geom(a.g, d->in, d->out, d->boundary, d->const_val, d->cvt, 0.0f);
+  if (st) return st;
+  if (!((d->in.dtype == HIPACC_F32 && d->out.dtype == HIPACC_F32) ||
+        (d->in.dtype == HIPACC_U8 && d->out.dtype == HIPACC_U8)))
+    return set_error(HIPACC_ERR_UNSUPPORTED, "bilateral: F32->F32 or U8->U8 only");
+  if (d->ksize < 1 || d->ksize > HIPACC_MAX_MASK_DIM || !(d->ksize & 1))
+    return set_error(HIPACC_ERR_INVALID_ARG, "bilateral: ksize %d", d->ksize);
+  if (!(d->sigma_r > 0.0f)) return set_error(HIPACC_ERR_INVALID_ARG, "bilateral: sigma_r must be > 0");
+  if (!d->spatial && !(d->sigma_d > 0.0f))
+    return set_error(HIPACC_ERR_INVALID_ARG, "bilateral: sigma_d must be > 0");
+  a.ksize = d->ksize;
+  const int r = d->ksize / 2;
+  for (int dy = 0; dy < d->ksize; ++dy)
+    for (int dx = 0; dx < d->ksize; ++dx) {
+      const int t = dy * d->ksize + dx;
+      float w;
+      if (d->spatial) {
+        w = d->spatial[t];
+      } else {
+        const double ddx = dx - r, ddy = dy - r;
+        w = (float)std::exp(-(ddx * ddx + ddy * ddy) / (2.0 * (double)d->sigma_d * d->sigma_d));
+      }
+      a.lsp[t] = w > 0.0f ? std::log2(w) : -INFINITY;
+    }
+  const float c_r = 0.5f / (d->sigma_r * d->sigma_r);
+  a.k2 = -c_r * 1.44269504088896340736f;
+  a.in_vec_ok = d->in.dtype == HIPACC_F32 && rows_aligned(d->in, 16) && (d->in.offset_x % 4) == 0;
+  a.out_vec_ok = rows_aligned(d->out, 4 * dtype_bytes(d->out.dtype));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Timer tm(s);
+  return tm.finish(launch_bilateral(a, s));
+}
+
+int hipacc_hip_alloc2d(void** ptr, int32_t* stride_elems, int32_t width, int32_t height,
+                       int32_t elem_bytes, int32_t alignment) {
+  if (!ptr || !stride_elems || width <= 0 || height <= 0 ||
+      !(elem_bytes == 1 || elem_bytes == 2 || elem_bytes == 4))
+    return set_error(HIPACC_ERR_INVALID_ARG, "alloc2d: bad arguments");
+  if (alignment <= 0) alignment = 256;
+  if (alignment % elem_bytes) return set_error(HIPACC_ERR_INVALID_ARG, "alloc2d: alignment");
+  const int per = alignment / elem_bytes;
+  const long long stride = ((long long)width + per - 1) / per * per;
+  *stride_elems = (int32_t)stride;
+  return hip_status(hipMalloc(ptr, (size_t)stride * height * elem_bytes), "hipMalloc");
+}
+
+int hipacc_hip_free(void* ptr) { return hip_status(hipFree(ptr), "hipFree"); }
+
+int hipacc_hip_upload2d(void* dst_dev, int32_t dst_stride_elems, const void* src_host, int32_t width,
+                        int32_t height, int32_t elem_bytes, void* stream) {
+  if (!dst_dev || !src_host) return set_error(HIPACC_ERR_INVALID_ARG, "upload2d: NULL pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = hipMemcpy2DAsync(dst_dev, (size_t)dst_stride_elems * elem_bytes, src_host,
+                                  (size_t)width * elem_bytes, (size_t)width * elem_bytes, height,
+                                  hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return hip_status(e, "upload2d");
+}
+
+int hipacc_hip_download2d(void* dst_host, const void* src_dev, int32_t src_stride_elems, int32_t width,
+                          int32_t height, int32_t elem_bytes, void* stream) {
+  if (!dst_host || !src_dev) return set_error(HIPACC_ERR_INVALID_ARG, "download2d: NULL pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = hipMemcpy2DAsync(dst_host, (size_t)width * elem_bytes, src_dev,
+                                  (size_t)src_stride_elems * elem_bytes, (size_t)width * elem_bytes,
+                                  height, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return hip_status(e, "download2d");
+}
+
+int hipacc_hip_copy_region(const hipacc_view* src, const hipacc_view* dst, void* stream) {
+  if (!src || !dst) return set_error(HIPACC_ERR_INVALID_ARG, "copy_region: NULL argument");
+  int st = check_view(*src, "src");
+  if (st) return st;
+  if ((st = check_view(*dst, "dst"))) return st;
+  if (src->dtype != dst->dtype || src->width != dst->width || src->height != dst->height)
+    return set_error(HIPACC_ERR_INVALID_ARG, "copy_region: windows differ in type or size");
+  const int eb = dtype_bytes(src->dtype);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  for (int b = 0; b < batch_of(*src); ++b) {
+    const char* sp = static_cast<const char*>(src->data) +
+                     ((long long)b * src->plane_stride + (long long)src->offset_y * src->stride + src->offset_x) * eb;
+    char* dp = static_cast<char*>(dst->data) +
+               ((long long)b * dst->plane_stride + (long long)dst->offset_y * dst->stride + dst->offset_x) * eb;
+    hipError_t e = hipMemcpy2DAsync(dp, (size_t)dst->stride * eb, sp, (size_t)src->stride * eb,
+                                    (size_t)src->width * eb, src->height, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return hip_status(e, "copy_region");
+  }
+  return HIPACC_OK;
+}
+
+int hipacc_hip_fill(void* dst_dev, uint32_t value, size_t count, int32_t elem_bytes, void* stream) {
+  if (!dst_dev) return set_error(HIPACC_ERR_INVALID_ARG, "fill: NULL pointer");
+  return launch_fill(dst_dev, value, count, elem_bytes, static_cast<hipStream_t>(stream));
+}
+
+int hipacc_hip_fill_random(const hipacc_view* img, uint64_t seed, void* stream) {
+  if (!img) return set_error(HIPACC_ERR_INVALID_ARG, "fill_random: NULL argument");
+  int st = check_view(*img, "img");
+  if (st) return st;
+  return launch_random(img->data, img->dtype, img->stride, img->offset_x, img->offset_y, img->width,
+                       img->height, batch_of(*img), img->plane_stride, seed,
+                       static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,278 @@
+// reduce.hip -- global operators: reduced_data() and binned_data(), plus the
+// small utility kernels (fill, synthetic pixels, window copy).
+//
+// Reduction: a fixed grid of blocks walks the window's rows (block b takes
+// rows b, b+G, ...; threads stride along the row with 16-byte loads when the
+// row is aligned).  Each wave folds its 64 lanes with cross-lane shuffles,
+// one LDS level folds the 4 waves, and block partials go to the workspace.
+// The block that takes the last ticket folds the partials in a fixed order,
+// writes the result and re-arms the ticket, so one launch does the whole
+// reduction and the result does not depend on scheduling.
+#include "hh_tile.hpp"
+
+namespace hh {
+
+template <typename A> __device__ __forceinline__ A red_identity(int mode);
+template <> __device__ __forceinline__ float red_identity<float>(int mode) {
+  switch (mode) {
+    case HIPACC_REDUCE_MIN: return __builtin_huge_valf();
+    case HIPACC_REDUCE_MAX: return -__builtin_huge_valf();
+    case HIPACC_REDUCE_PROD: return 1.0f;
+    default: return 0.0f;
+  }
+}
+template <> __device__ __forceinline__ uint32_t red_identity<uint32_t>(int mode) {
+  switch (mode) {
+    case HIPACC_REDUCE_MIN: return 0xffffffffu;
+    case HIPACC_REDUCE_MAX: return 0u;
+    case HIPACC_REDUCE_PROD: return 1u;
+    default: return 0u;
+  }
+}
+template <> __device__ __forceinline__ int32_t red_identity<int32_t>(int mode) {
+  switch (mode) {
+    case HIPACC_REDUCE_MIN: return 0x7fffffff;
+    case HIPACC_REDUCE_MAX: return (int32_t)0x80000000;
+    case HIPACC_REDUCE_PROD: return 1;
+    default: return 0;
+  }
+}
+
+template <typename A> __device__ __forceinline__ A red_op(int mode, A a, A b) {
+  switch (mode) {
+    case HIPACC_REDUCE_MIN: return b < a ? b : a;
+    case HIPACC_REDUCE_MAX: return b > a ? b : a;
+    case HIPACC_REDUCE_PROD: return a * b;
+    default: return a + b;
+  }
+}
+
+template <typename A> __device__ __forceinline__ A wave_fold(int mode, A v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = red_op<A>(mode, v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+template <typename A> __device__ __forceinline__ A block_fold(int mode, A v, A* sm) {
+  v = wave_fold<A>(mode, v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) sm[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    v = sm[0];
+    for (int k = 1; k < 4; ++k) v = red_op<A>(mode, v, sm[k]);
+  }
+  return v;
+}
+
+template <typename T, typename A>
+__global__ __launch_bounds__(256) void reduce_kernel(const ReduceArgs a) {
+  constexpr int VEC = 16 / (int)sizeof(T);
+  __shared__ A sm[4];
+  __shared__ bool last;
+  const int mode = a.mode;
+  A acc = red_identity<A>(mode);
+  for (int y = blockIdx.x; y < a.h; y += gridDim.x) {
+    const T* row = static_cast<const T*>(a.in) + (long long)(a.oy + y) * a.stride + a.ox;
+    if (a.vec_ok) {
+      const int nv = a.w / VEC;
+      for (int i = threadIdx.x; i < nv; i += 256) {
+        const uint4 w = reinterpret_cast<const uint4*>(row)[i];
+        T t[VEC];
+        __builtin_memcpy(t, &w, 16);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) acc = red_op<A>(mode, acc, (A)t[k]);
+      }
+      for (int x = nv * VEC + threadIdx.x; x < a.w; x += 256) acc = red_op<A>(mode, acc, (A)row[x]);
+    } else {
+      for (int x = threadIdx.x; x < a.w; x += 256) acc = red_op<A>(mode, acc, (A)row[x]);
+    }
+  }
+  acc = block_fold<A>(mode, acc, sm);
+  if (threadIdx.x == 0) {
+    unsigned long long bits = 0;
+    __builtin_memcpy(&bits, &acc, sizeof(A));
+    a.partials[blockIdx.x] = bits;
+    __threadfence();
+    const unsigned int t = atomicAdd(a.ticket, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  A v = red_identity<A>(mode);
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += 256) {
+    const unsigned long long bits = __atomic_load_n(&a.partials[i], __ATOMIC_RELAXED);
+    A p;
+    __builtin_memcpy(&p, &bits, sizeof(A));
+    v = red_op<A>(mode, v, p);
+  }
+  __syncthreads();
+  v = block_fold<A>(mode, v, sm);
+  if (threadIdx.x == 0) {
+    *static_cast<A*>(a.result) = v;
+    *a.ticket = 0u;  // re-arm for the next launch on this workspace
+  }
+}
+
+template <typename T, typename A>
+static int launch_reduce_t(const ReduceArgs& a, hipStream_t s) {
+  const int blocks = a.h < kRedBlocks ? a.h : kRedBlocks;
+  hipLaunchKernelGGL((reduce_kernel<T, A>), dim3(blocks), dim3(256), 0, s, a);
+  return hip_status(hipGetLastError(), "reduce_kernel launch");
+}
+
+// acc type: float for F32 images; for integer images the requested
+// accumulator (U32 or I32) -- min/max of integer images are exact in either.
+int launch_reduce(const ReduceArgs& a, int dt, int acc_dt, hipStream_t s) {
+  const bool ai = acc_dt == HIPACC_I32;
+  switch (dt) {
+    case HIPACC_F32:
+      if (acc_dt != HIPACC_F32) break;
+      return launch_reduce_t<float, float>(a, s);
+    case HIPACC_U8: return ai ? launch_reduce_t<uint8_t, int32_t>(a, s) : launch_reduce_t<uint8_t, uint32_t>(a, s);
+    case HIPACC_I8: return ai ? launch_reduce_t<int8_t, int32_t>(a, s) : launch_reduce_t<int8_t, uint32_t>(a, s);
+    case HIPACC_U16: return ai ? launch_reduce_t<uint16_t, int32_t>(a, s) : launch_reduce_t<uint16_t, uint32_t>(a, s);
+    case HIPACC_I16: return ai ? launch_reduce_t<int16_t, int32_t>(a, s) : launch_reduce_t<int16_t, uint32_t>(a, s);
+    case HIPACC_U32: return ai ? launch_reduce_t<uint32_t, int32_t>(a, s) : launch_reduce_t<uint32_t, uint32_t>(a, s);
+    case HIPACC_I32: return ai ? launch_reduce_t<int32_t, int32_t>(a, s) : launch_reduce_t<int32_t, uint32_t>(a, s);
+  }
+  return set_error(HIPACC_ERR_UNSUPPORTED, "reduce: image/accumulator type combination");
+}
+
+// ---------------------------------------------------------------------------
+// Histogram: LDS-privatised bins per block (ds_add_u32), merged into the
+// global bins with one atomic per non-empty bin per block.
+// ---------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ int bin_of(T p, int nbins, float scale);
+template <> __device__ __forceinline__ int bin_of<uint8_t>(uint8_t p, int nbins, float) {
+  return (int)(((uint32_t)p * (uint32_t)nbins) >> 8);
+}
+template <> __device__ __forceinline__ int bin_of<uint16_t>(uint16_t p, int nbins, float) {
+  return (int)(((uint32_t)p * (uint32_t)nbins) >> 16);
+}
+template <> __device__ __forceinline__ int bin_of<float>(float p, int nbins, float scale) {
+  const float f = p * scale;
+  if (!(f >= 0.0f) || f >= (float)nbins) return -1;
+  return (int)(uint32_t)f;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void histogram_kernel(const HistArgs a) {
+  constexpr int VEC = 16 / (int)sizeof(T);
+  extern __shared__ uint32_t hist[];
+  for (int i = threadIdx.x; i < a.nbins; i += 256) hist[i] = 0u;
+  __syncthreads();
+  for (int y = blockIdx.x; y < a.h; y += gridDim.x) {
+    const T* row = static_cast<const T*>(a.in) + (long long)(a.oy + y) * a.stride + a.ox;
+    int start = 0;
+    if (a.vec_ok) {
+      const int nv = a.w / VEC;
+      for (int i = threadIdx.x; i < nv; i += 256) {
+        const uint4 w = reinterpret_cast<const uint4*>(row)[i];
+        T t[VEC];
+        __builtin_memcpy(t, &w, 16);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+          const int b = bin_of<T>(t[k], a.nbins, a.scale);
+          if (b >= 0 && b < a.nbins) atomicAdd(&hist[b], 1u);
+        }
+      }
+      start = nv * VEC;
+    }
+    for (int x = start + threadIdx.x; x < a.w; x += 256) {
+      const int b = bin_of<T>(row[x], a.nbins, a.scale);
+      if (b >= 0 && b < a.nbins) atomicAdd(&hist[b], 1u);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < a.nbins; i += 256)
+    if (hist[i]) atomicAdd(&a.bins[i], hist[i]);
+}
+
+int launch_histogram(const HistArgs& a, int dt, hipStream_t s) {
+  const int blocks = a.h < kRedBlocks ? a.h : kRedBlocks;
+  const size_t smem = (size_t)a.nbins * 4;
+  switch (dt) {
+    case HIPACC_U8:
+      hipLaunchKernelGGL(histogram_kernel<uint8_t>, dim3(blocks), dim3(256), smem, s, a); break;
+    case HIPACC_U16:
+      hipLaunchKernelGGL(histogram_kernel<uint16_t>, dim3(blocks), dim3(256), smem, s, a); break;
+    case HIPACC_F32:
+      hipLaunchKernelGGL(histogram_kernel<float>, dim3(blocks), dim3(256), smem, s, a); break;
+    default:
+      return set_error(HIPACC_ERR_UNSUPPORTED, "histogram: pixel type must be U8, U16 or F32");
+  }
+  return hip_status(hipGetLastError(), "histogram_kernel launch");
+}
+
+// ---------------------------------------------------------------------------
+// Utilities
+// ---------------------------------------------------------------------------
+template <typename W>
+__global__ void fill_kernel(W* p, W v, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = v;
+}
+
+int launch_fill(void* p, uint32_t value, size_t n, int eb, hipStream_t s) {
+  if (n == 0) return HIPACC_OK;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  switch (eb) {
+    case 1: hipLaunchKernelGGL(fill_kernel<uint8_t>, dim3(blocks), dim3(256), 0, s, (uint8_t*)p, (uint8_t)value, n); break;
+    case 2: hipLaunchKernelGGL(fill_kernel<uint16_t>, dim3(blocks), dim3(256), 0, s, (uint16_t*)p, (uint16_t)value, n); break;
+    case 4: hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(blocks), dim3(256), 0, s, (uint32_t*)p, value, n); break;
+    default: return set_error(HIPACC_ERR_INVALID_ARG, "fill: elem_bytes must be 1, 2 or 4");
+  }
+  return hip_status(hipGetLastError(), "fill_kernel launch");
+}
+
+// SplitMix64 (Steele, Lea, Flood 2014): the k-th output of a generator seeded
+// with s is mix(s + k * 0x9e3779b97f4a7c15), k >= 1.
+HH_HD uint64_t splitmix64_at(uint64_t seed, uint64_t k) {
+  uint64_t z = seed + k * 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+template <typename T> __device__ __forceinline__ T rand_pixel(uint64_t x);
+template <> __device__ __forceinline__ uint8_t rand_pixel<uint8_t>(uint64_t x) { return (uint8_t)(x >> 56); }
+template <> __device__ __forceinline__ int8_t rand_pixel<int8_t>(uint64_t x) { return (int8_t)(x >> 56); }
+template <> __device__ __forceinline__ uint16_t rand_pixel<uint16_t>(uint64_t x) { return (uint16_t)(x >> 48); }
+template <> __device__ __forceinline__ int16_t rand_pixel<int16_t>(uint64_t x) { return (int16_t)(x >> 48); }
+template <> __device__ __forceinline__ uint32_t rand_pixel<uint32_t>(uint64_t x) { return (uint32_t)(x >> 32); }
+template <> __device__ __forceinline__ int32_t rand_pixel<int32_t>(uint64_t x) { return (int32_t)(x >> 32); }
+template <> __device__ __forceinline__ float rand_pixel<float>(uint64_t x) {
+  return (float)(x >> 40) * (1.0f / 16777216.0f);
+}
+
+template <typename T>
+__global__ void random_kernel(T* p, int stride, int ox, int oy, int w, int h, long long plane,
+                              uint64_t seed) {
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= w) return;
+  const uint64_t k = (uint64_t)blockIdx.z * (uint64_t)w * (uint64_t)h + (uint64_t)y * w + x + 1;
+  p[(long long)blockIdx.z * plane + (long long)(oy + y) * stride + ox + x] = rand_pixel<T>(splitmix64_at(seed, k));
+}
+
+int launch_random(void* p, int dt, int stride, int ox, int oy, int w, int h, int batch, long long plane,
+                  uint64_t seed, hipStream_t s) {
+  dim3 grid(ceil_div(w, 256), h, batch);
+#define HH_RND(T) hipLaunchKernelGGL(random_kernel<T>, grid, dim3(256), 0, s, (T*)p, stride, ox, oy, w, h, plane, seed)
+  switch (dt) {
+    case HIPACC_U8: HH_RND(uint8_t); break;
+    case HIPACC_I8: HH_RND(int8_t); break;
+    case HIPACC_U16: HH_RND(uint16_t); break;
+    case HIPACC_I16: HH_RND(int16_t); break;
+    case HIPACC_U32: HH_RND(uint32_t); break;
+    case HIPACC_I32: HH_RND(int32_t); break;
+    case HIPACC_F32: HH_RND(float); break;
+    default: return set_error(HIPACC_ERR_INVALID_ARG, "fill_random: bad dtype");
+  }
+#undef HH_RND
+  return hip_status(hipGetLastError(), "random_kernel launch");
+}
+
+}  // namespace hh

@@ -1,0 +1,239 @@
+"""Host-side calls of the C ABI on torch device tensors.
+
+Images are 2D tensors (H, W) -- row pitch taken from ``tensor.stride(0)`` -- or
+3D tensors (B, H, W) for batches.  Each function builds the descriptor of
+include/hipacc_hip.h and launches on torch's current stream.  Windows
+(accessor regions / iteration spaces) are given as ``(x, y, w, h)``; the
+default is the whole image.
+
+The operator vocabulary is Hipacc's: ``local`` is Kernel::convolve /
+reduce(Domain) / iterate, ``point`` is a point operator's output(),
+``reduce`` is reduced_data(), ``histogram`` is binned_data() of a counting
+binning, ``bilateral`` is the Bilateral_Filter operator.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import capi
+
+_TORCH_DTYPES = {
+    torch.uint8: capi.U8, torch.int8: capi.I8, torch.int16: capi.I16, torch.int32: capi.I32,
+    torch.float32: capi.F32,
+}
+for _name, _code in (("uint16", capi.U16), ("uint32", capi.U32)):
+    if hasattr(torch, _name):
+        _TORCH_DTYPES[getattr(torch, _name)] = _code
+
+Window = Optional[Tuple[int, int, int, int]]
+
+
+def dtype_code(t: torch.Tensor) -> int:
+    try:
+        return _TORCH_DTYPES[t.dtype]
+    except KeyError:
+        raise TypeError(f"unsupported pixel dtype {t.dtype}") from None
+
+
+def view(t: torch.Tensor, window: Window = None, dtype: Optional[int] = None) -> capi.View:
+    """hipacc_view of a (H, W) or (B, H, W) device tensor, rows contiguous."""
+    if not t.is_cuda:
+        raise ValueError("image tensors must live on the GPU")
+    if t.dim() not in (2, 3) or t.stride(-1) != 1:
+        raise ValueError("images are (H, W) or (B, H, W) tensors with unit column stride")
+    h, w = t.shape[-2], t.shape[-1]
+    v = capi.View()
+    v.data = t.data_ptr()
+    v.dtype = dtype_code(t) if dtype is None else dtype
+    v.stride = t.stride(-2)
+    v.img_width, v.img_height = w, h
+    x, y, ww, hh = window if window is not None else (0, 0, w, h)
+    v.offset_x, v.offset_y, v.width, v.height = x, y, ww, hh
+    v.batch = t.shape[0] if t.dim() == 3 else 1
+    v.plane_stride = t.stride(0) if t.dim() == 3 else 0
+    return v
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+class Call:
+    """A prepared C-ABI call: the descriptor is built once (host tables kept
+    alive here) and ``launch(stream)`` only crosses the ABI."""
+
+    def __init__(self, fn_name: str, desc, keep=()):
+        self.fn = getattr(capi.lib(), fn_name)
+        self.name = fn_name
+        self.desc = desc
+        self.ref = ctypes.byref(desc)
+        self.keep = list(keep)
+
+    def launch(self, stream: Optional[int] = None) -> None:
+        st = self.fn(self.ref, _stream() if stream is None else stream)
+        if st:
+            capi.check(st, self.name)
+
+
+def _arr(values, np_dtype):
+    a = np.ascontiguousarray(np.asarray(values, dtype=np_dtype).reshape(-1))
+    return a, a.ctypes.data
+
+
+def local(*args, **kw) -> None:
+    prepare_local(*args, **kw).launch()
+
+
+def prepare_local(src: torch.Tensor, dst: torch.Tensor, mask_w: int, mask_h: int, *,
+          weights: Optional[Sequence[float]] = None, weights_dev: Optional[torch.Tensor] = None,
+          domain: Optional[Sequence[int]] = None, acc_float: bool = True,
+          reduce: int = capi.SUM, boundary: int = capi.CLAMP, const_val: float = 0.0,
+          cvt: int = capi.CVT_TRUNC, post_div: float = 0.0, fp_contract: bool = True,
+          src_window: Window = None, dst_window: Window = None,
+          dst_dtype: Optional[int] = None) -> Call:
+    d = capi.LocalDesc()
+    d.input = view(src, src_window)
+    d.output = view(dst, dst_window, dst_dtype)
+    d.mask_w, d.mask_h = mask_w, mask_h
+    keep = []
+    if weights is not None:
+        if acc_float:
+            a, p = _arr(weights, np.float32)
+            d.weights_f32 = p
+        else:
+            a, p = _arr(weights, np.int32)
+            d.weights_i32 = p
+        keep.append(a)
+    if weights_dev is not None:
+        d.weights_dev = weights_dev.data_ptr()
+    d.use_weights = int(weights is not None or weights_dev is not None)
+    if domain is not None:
+        a, p = _arr(domain, np.uint8)
+        d.domain = p
+        keep.append(a)
+    d.acc_dtype = capi.F32 if acc_float else capi.I32
+    d.reduce, d.boundary, d.const_val = reduce, boundary, const_val
+    d.cvt, d.post_div, d.fp_contract = cvt, post_div, int(fp_contract)
+    return Call("hipacc_hip_local", d, keep)
+
+
+def separable(*args, **kw) -> None:
+    prepare_separable(*args, **kw).launch()
+
+
+def prepare_separable(src: torch.Tensor, dst: torch.Tensor, row: Sequence, col: Sequence, *,
+              acc_float: bool = True, boundary: int = capi.CLAMP, const_val: float = 0.0,
+              cvt: int = capi.CVT_TRUNC, post_div: float = 0.0, fp_contract: bool = True,
+              src_window: Window = None, dst_window: Window = None,
+              dst_dtype: Optional[int] = None) -> Call:
+    d = capi.SeparableDesc()
+    d.input = view(src, src_window)
+    d.output = view(dst, dst_window, dst_dtype)
+    d.ksize_x, d.ksize_y = len(row), len(col)
+    if acc_float:
+        ra, rp = _arr(row, np.float32)
+        ca, cp = _arr(col, np.float32)
+        d.row_f32, d.col_f32 = rp, cp
+    else:
+        ra, rp = _arr(row, np.int32)
+        ca, cp = _arr(col, np.int32)
+        d.row_i32, d.col_i32 = rp, cp
+    d.acc_dtype = capi.F32 if acc_float else capi.I32
+    d.boundary, d.const_val, d.cvt, d.post_div = boundary, const_val, cvt, post_div
+    d.fp_contract = int(fp_contract)
+    return Call("hipacc_hip_separable", d, (ra, ca))
+
+
+def point(*args, **kw) -> None:
+    prepare_point(*args, **kw).launch()
+
+
+def prepare_point(op: int, src0: torch.Tensor, dst: torch.Tensor,
+                  src1: Optional[torch.Tensor] = None, *,
+          alpha: float = 1.0, beta: float = 0.0, cvt: int = capi.CVT_TRUNC,
+          windows: Tuple[Window, Window, Window] = (None, None, None),
+          dst_dtype: Optional[int] = None) -> Call:
+    d = capi.PointDesc()
+    d.in0 = view(src0, windows[0])
+    if src1 is not None:
+        d.in1 = view(src1, windows[1])
+    d.output = view(dst, windows[2], dst_dtype)
+    d.op, d.alpha, d.beta, d.cvt = op, alpha, beta, cvt
+    return Call("hipacc_hip_point", d)
+
+
+class ReduceCall:
+    """Prepared hipacc_hip_reduce (reduced_data() into a device scalar)."""
+
+    def __init__(self, src, mode, result, acc_dtype=None, window=None, workspace=None):
+        self.v = view(src, window)
+        if acc_dtype is None:
+            acc_dtype = capi.F32 if self.v.dtype == capi.F32 else capi.U32
+        self.args = (ctypes.byref(self.v), mode, acc_dtype, result.data_ptr(),
+                     workspace.data_ptr() if workspace is not None else None)
+        self.fn = capi.lib().hipacc_hip_reduce
+
+    def launch(self, stream: Optional[int] = None) -> None:
+        st = self.fn(*self.args, _stream() if stream is None else stream)
+        if st:
+            capi.check(st, "hipacc_hip_reduce")
+
+
+def reduce(src: torch.Tensor, mode: int, result: torch.Tensor, *, acc_dtype: Optional[int] = None,
+           window: Window = None, workspace: Optional[torch.Tensor] = None) -> None:
+    """Writes reduced_data() of the window into the one-element device tensor
+    `result` (asynchronous)."""
+    ReduceCall(src, mode, result, acc_dtype, window, workspace).launch()
+
+
+class HistogramCall:
+    """Prepared hipacc_hip_histogram."""
+
+    def __init__(self, src, num_bins, bins, scale=1.0, window=None):
+        self.v = view(src, window)
+        self.args = (ctypes.byref(self.v), num_bins, scale, bins.data_ptr())
+        self.fn = capi.lib().hipacc_hip_histogram
+
+    def launch(self, stream: Optional[int] = None) -> None:
+        st = self.fn(*self.args, _stream() if stream is None else stream)
+        if st:
+            capi.check(st, "hipacc_hip_histogram")
+
+
+def histogram(src: torch.Tensor, num_bins: int, bins: torch.Tensor, *, scale: float = 1.0,
+              window: Window = None) -> None:
+    HistogramCall(src, num_bins, bins, scale, window).launch()
+
+
+def bilateral(*args, **kw) -> None:
+    prepare_bilateral(*args, **kw).launch()
+
+
+def prepare_bilateral(src: torch.Tensor, dst: torch.Tensor, ksize: int, sigma_d: float,
+                      sigma_r: float, *,
+              spatial: Optional[Sequence[float]] = None, boundary: int = capi.MIRROR,
+              const_val: float = 0.0, cvt: int = capi.CVT_TRUNC, src_window: Window = None,
+              dst_window: Window = None) -> Call:
+    d = capi.BilateralDesc()
+    d.input = view(src, src_window)
+    d.output = view(dst, dst_window)
+    d.ksize, d.sigma_d, d.sigma_r = ksize, sigma_d, sigma_r
+    keep = None
+    if spatial is not None:
+        keep, d.spatial = _arr(spatial, np.float32)
+    d.boundary, d.const_val, d.cvt = boundary, const_val, cvt
+    return Call("hipacc_hip_bilateral", d, [keep])
+
+
+def fill_random(t: torch.Tensor, seed: int, window: Window = None) -> None:
+    v = view(t, window)
+    capi.check(capi.lib().hipacc_hip_fill_random(ctypes.byref(v), seed, _stream()),
+               "hipacc_hip_fill_random")
+
+
+def init(device: int = 0) -> None:
+    capi.check(capi.lib().hipacc_hip_init(device), "hipacc_hip_init")

@@ -1,0 +1,308 @@
+/*
+ * hipacc_hip.h -- C ABI of libhipacc_gfx950.so, the MI355X (gfx950) execution
+ * library for Hipacc's data-parallel operators.
+ *
+ * The reference executes DSL operators through generated kernels launched by
+ * its CUDA runtime (runtime/hipacc_cu.hpp, runtime/hipacc_cu.tpp,
+ * runtime/hipacc_cu_standalone.hpp, runtime/hipacc_cu_red.hpp).  This library
+ * replaces the generated-kernel layer: instead of one generated kernel per DSL
+ * operator, the host side describes the operator (accessor window, iteration
+ * space, mask/domain, reduction mode, boundary mode, output conversion) and
+ * calls one of the entry points below, which launch hand-written gfx950
+ * kernels.
+ *
+ * Conventions
+ *  - Every entry point returns an int status (HIPACC_OK == 0) and never
+ *    aborts; hipacc_hip_strerror() names the code, hipacc_hip_last_error()
+ *    gives the detailed message of the calling thread's last failure.
+ *  - Pixel pointers are device pointers; mask / domain tables are host
+ *    pointers (copied into the kernel arguments at launch), except the
+ *    explicitly named *_dev members.
+ *  - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *    Launches are asynchronous on that stream; nothing here synchronises
+ *    unless its comment says so.
+ *  - Calls on different streams may run concurrently; calls that use the
+ *    library's internal workspace (reductions, histograms, weight gradients)
+ *    serialise that workspace per device with a mutex at enqueue time only,
+ *    so concurrent calls on different streams must pass their own workspace.
+ */
+#ifndef HIPACC_HIP_H
+#define HIPACC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  HIPACC_OK = 0,
+  HIPACC_ERR_INVALID_ARG = 1,   /* a descriptor field is out of range      */
+  HIPACC_ERR_UNSUPPORTED = 2,   /* valid, but not implemented by a kernel  */
+  HIPACC_ERR_HIP = 3,           /* a HIP runtime call failed               */
+  HIPACC_ERR_NO_DEVICE = 4,     /* no gfx950 device / init failed          */
+  HIPACC_ERR_OUT_OF_MEMORY = 5
+};
+
+/* ---- pixel types ------------------------------------------------------- */
+/* The scalar pixel types of the DSL (dsl/types.hpp:33-36 uchar/ushort/uint
+ * plus char/short/int/float).                                              */
+typedef enum {
+  HIPACC_U8 = 0, HIPACC_I8 = 1, HIPACC_U16 = 2, HIPACC_I16 = 3,
+  HIPACC_U32 = 4, HIPACC_I32 = 5, HIPACC_F32 = 6
+} hipacc_dtype;
+
+/* ---- boundary modes: same values as hipacc::Boundary (dsl/image.hpp:46-52) */
+typedef enum {
+  HIPACC_BOUNDARY_UNDEFINED = 0,
+  HIPACC_BOUNDARY_CLAMP = 1,
+  HIPACC_BOUNDARY_REPEAT = 2,
+  HIPACC_BOUNDARY_MIRROR = 3,
+  HIPACC_BOUNDARY_CONSTANT = 4
+} hipacc_boundary;
+
+/* ---- reduction modes: same values as hipacc::Reduce (dsl/kernel.hpp:48-54);
+ * MEDIAN is unsupported there (dsl/kernel.hpp:259) and here.               */
+typedef enum {
+  HIPACC_REDUCE_SUM = 0, HIPACC_REDUCE_MIN = 1, HIPACC_REDUCE_MAX = 2,
+  HIPACC_REDUCE_PROD = 3
+} hipacc_reduce;
+
+/* ---- accumulator -> output conversion --------------------------------- */
+/* TRUNC is the C cast the DSL applies on `output() = acc`
+ * (dsl/types.hpp:115-117 convert<T>(float) == (T)s).  ROUND is the samples'
+ * idiom `(uchar)(acc + 0.5f)` (e.g. samples-public/1_Local_Operators/
+ * Gaussian_Blur/src/main.cpp:63-65).  Float-to-integer conversion truncates
+ * toward zero to a 64-bit integer (NaN -> 0, saturating at the int64 range)
+ * and then wraps to the output width, which is what x86-64 code does for every
+ * value inside the int32 range; the reference leaves the rest undefined.    */
+typedef enum { HIPACC_CVT_TRUNC = 0, HIPACC_CVT_ROUND = 1 } hipacc_cvt;
+
+/* ---- an image window: an Accessor or an IterationSpace ----------------- */
+/* Mirrors HipaccAccessor / hipacc_launch_info geometry
+ * (runtime/hipacc_base.hpp:72-101, runtime/hipacc_cu.hpp:162-174): the
+ * window [offset_x, offset_x+width) x [offset_y, offset_y+height) of an image
+ * of img_width x img_height pixels with a row pitch of `stride` elements.
+ * For an Accessor the window is the region boundary handling refers to
+ * (lib/AST/BorderHandling.cpp:155-168); for an IterationSpace it is the set
+ * of output pixels (dsl/iterationspace.hpp:45-138).
+ * `batch` images are laid out `plane_stride` elements apart (batch 0 or 1 ==
+ * a single image).                                                          */
+typedef struct hipacc_view {
+  void*   data;          /* device pointer to pixel (0,0) of image 0       */
+  int32_t dtype;         /* hipacc_dtype                                    */
+  int32_t stride;        /* row pitch, elements                             */
+  int32_t img_width;     /* allocation extent                               */
+  int32_t img_height;
+  int32_t width;         /* window                                          */
+  int32_t height;
+  int32_t offset_x;
+  int32_t offset_y;
+  int32_t batch;         /* number of images (0 == 1)                       */
+  int64_t plane_stride;  /* elements between consecutive images             */
+} hipacc_view;
+
+/* ---- local operator: Kernel::convolve / reduce(Domain) / iterate ------ */
+/* Computes, for every pixel p of the iteration space `out`,
+ *     acc = (+)_{taps t in row-major order, domain[t] != 0} tap(t)
+ *     tap(t) = weight[t] * in_bh(p + t)    (use_weights != 0)
+ *            = in_bh(p + t)                (use_weights == 0)
+ *     out(p) = cvt(post_div != 0 ? (float)acc / post_div : acc)
+ * with (+) the reduction `reduce`, seeded by the first active tap as in
+ * Kernel::convolve / Kernel::reduce (dsl/kernel.hpp:241-296), taps in the
+ * order of ElementIterator / DomainIterator (dsl/iterationspace.hpp:78-90,
+ * dsl/mask.hpp:112-126), and in_bh the accessor read with boundary handling
+ * (dsl/image.hpp:574-612; generated form lib/AST/BorderHandling.cpp:41-120).
+ * The input coordinate of output pixel (x, y) is
+ * (x - out.offset_x + in.offset_x, y - out.offset_y + in.offset_y)
+ * (dsl/image.hpp:390-412, Interpolate::NO).
+ * acc_dtype is HIPACC_F32 (float lambda, e.g. Gaussian) or HIPACC_I32
+ * (int lambda, e.g. Sobel / box sum).  fp_contract = 1 evaluates a float
+ * weighted SUM as acc = fma(w, x, acc), the contraction the reference CUDA
+ * backend gets from nvcc's default -fmad=true; 0 evaluates acc + w*x with
+ * two roundings, as the DSL host emulation compiled by g++ for x86-64 does.
+ * Replaces: one generated kernel per convolve() call site
+ * (lib/AST/Convolution.cpp:228-467, launched via hipaccLaunchKernel,
+ * runtime/hipacc_cu_standalone.hpp:277-329).                               */
+#define HIPACC_MAX_MASK_DIM 15
+typedef struct hipacc_local_desc {
+  hipacc_view in;            /* Accessor                                    */
+  hipacc_view out;           /* IterationSpace                              */
+  int32_t mask_w, mask_h;    /* odd, 1..HIPACC_MAX_MASK_DIM                 */
+  const float*   weights_f32;/* host, mask_h*mask_w, row-major (float mask) */
+  const int32_t* weights_i32;/* host, integer mask (acc_dtype I32)          */
+  const float*   weights_dev;/* device, float mask (learnable weights);
+                                takes precedence over the host tables       */
+  const uint8_t* domain;     /* host, nonzero = tap active; NULL = all      */
+  int32_t use_weights;       /* 0: tap = pixel (Domain reduce)              */
+  int32_t acc_dtype;         /* HIPACC_F32 or HIPACC_I32                    */
+  int32_t reduce;            /* hipacc_reduce                               */
+  int32_t boundary;          /* hipacc_boundary                             */
+  double  const_val;         /* CONSTANT value, in the input pixel type     */
+  int32_t cvt;               /* hipacc_cvt                                  */
+  float   post_div;          /* 0 = none                                    */
+  int32_t fp_contract;       /* see above                                   */
+} hipacc_local_desc;
+int hipacc_hip_local(const hipacc_local_desc* desc, void* stream);
+
+/* ---- separable local operator: a row pass then a column pass ---------- */
+/* The two-kernel DSL program
+ *     tmp(x, y) = SUM_i row[i] * in_bh(x + i - rx, y)          (float)
+ *     out(x, y) = cvt( SUM_j col[j] * tmp_bh(x, y + j - ry) )
+ * with tmp an Image<float> the size of the accessor window and both kernels
+ * using `boundary` (for CONSTANT, tmp rows outside the window read
+ * const_val), fused into one launch that keeps tmp in registers.
+ * acc_dtype HIPACC_I32 with integer row/col weights computes the same two
+ * passes in 32-bit integers (exact, so equal to the 2D integer mask
+ * row (x) col, e.g. a box sum).  The iteration space must have the size of
+ * the accessor window.  Replaces two generated convolve() kernels.         */
+typedef struct hipacc_separable_desc {
+  hipacc_view in;
+  hipacc_view out;
+  int32_t ksize_x, ksize_y;   /* odd, 1..2*HIPACC_MAX_SEP_RADIUS+1          */
+  const float*   row_f32;     /* host, ksize_x weights                      */
+  const float*   col_f32;     /* host, ksize_y weights                      */
+  const int32_t* row_i32;     /* host, integer weights (acc_dtype I32)      */
+  const int32_t* col_i32;
+  int32_t acc_dtype;
+  int32_t boundary;
+  double  const_val;
+  int32_t cvt;
+  float   post_div;
+  int32_t fp_contract;
+} hipacc_separable_desc;
+#define HIPACC_MAX_SEP_RADIUS 12
+int hipacc_hip_separable(const hipacc_separable_desc* desc, void* stream);
+
+/* ---- point operator: Kernel::execute with a per-pixel output() -------- */
+/* Per pixel of the iteration space `out` (dsl/kernel.hpp:94-119,
+ * dsl/iterationspace.hpp:78-90), reading in0 (and in1) at the same
+ * iteration-space-relative coordinate (no boundary handling):
+ *   COPY     out = cvt(a)                 AFFINE  out = cvt(alpha*a + beta)
+ *   ADD      out = cvt(a + b)             SUB     out = cvt(a - b)
+ *   MUL      out = cvt(a * b)             ABSDIFF out = cvt(|a - b|)
+ *   MIN/MAX  out = cvt(min/max(a, b))
+ *   MAGNITUDE out = cvt(clamp(sqrtf((float)(a'*a' + b'*b')), 0, 255)) with
+ *            a' = a / (int)alpha for integer inputs (C division), as the
+ *            Sobel combine kernel (samples-public/3_Preprocessing/Sobel/
+ *            src/main.cpp:87-96).
+ * Arithmetic is int32 when both inputs are integer types, float otherwise,
+ * matching C++'s usual arithmetic conversions for these types.            */
+typedef enum {
+  HIPACC_POINT_COPY = 0, HIPACC_POINT_AFFINE = 1, HIPACC_POINT_ADD = 2,
+  HIPACC_POINT_SUB = 3, HIPACC_POINT_MUL = 4, HIPACC_POINT_ABSDIFF = 5,
+  HIPACC_POINT_MIN = 6, HIPACC_POINT_MAX = 7, HIPACC_POINT_MAGNITUDE = 8
+} hipacc_point_op;
+typedef struct hipacc_point_desc {
+  hipacc_view in0;
+  hipacc_view in1;            /* data == NULL for unary ops                  */
+  hipacc_view out;
+  int32_t op;
+  float   alpha, beta;
+  int32_t cvt;
+} hipacc_point_desc;
+int hipacc_hip_point(const hipacc_point_desc* desc, void* stream);
+
+/* ---- global reduction: Kernel::reduced_data() ------------------------- */
+/* Reduces the window of `img` with `reduce` (SUM/MIN/MAX/PROD) and writes
+ * one value of type acc_dtype to the device pointer `result_dev`.  SUM of an
+ * integer image accumulates in acc_dtype (e.g. U8 image, U32 sum: the DSL
+ * program with a Kernel<uint> point stage).  Float SUM is evaluated as a
+ * fixed-shape tree (deterministic run to run); the reference's own CPU
+ * result depends on the OpenMP thread count (runtime/hipacc_cpu_red.hpp:
+ * 19-68), so float sums agree with it only to a tolerance.
+ * workspace: device memory of hipacc_hip_reduce_workspace_bytes() bytes, or
+ * NULL to use the library's per-device workspace.
+ * Replaces hipaccApplyReductionShared (runtime/hipacc_cu.tpp:311-408) and
+ * the kernel hipacc_shared_reduction (runtime/hipacc_cu_red.hpp:336-346).  */
+size_t hipacc_hip_reduce_workspace_bytes(void);
+int hipacc_hip_reduce(const hipacc_view* img, int32_t reduce, int32_t acc_dtype,
+                      void* result_dev, void* workspace, void* stream);
+/* Blocking convenience form: result copied to host memory `result_host`
+ * (acc_dtype-sized), as the reference's reduced_data() returns by value.   */
+int hipacc_hip_reduce_sync(const hipacc_view* img, int32_t reduce,
+                           int32_t acc_dtype, void* result_host, void* stream);
+
+/* ---- binning: Kernel::binned_data() for a counting histogram ---------- */
+/* bin(idx) = 1 with reduce(a, b) = a + b (samples-public/2_Global_Operators/
+ * Histogram/src/main.cpp:45-66).  idx = (p * num_bins) >> 8 for U8, >> 16
+ * for U16; idx = (uint)(p * scale) for F32 (p < 0 or NaN dropped).  Indices
+ * >= num_bins are dropped, as the generated CPU code does
+ * (runtime/hipacc_cpu_red.hpp:71-76).  bins_dev (device, num_bins uint32)
+ * is overwritten.  Replaces hipaccApplyBinningSegmented
+ * (runtime/hipacc_cu.tpp:410-464) and hipacc_binning_reduction
+ * (runtime/hipacc_cu_red.hpp:495-641).                                      */
+#define HIPACC_MAX_BINS 4096
+int hipacc_hip_histogram(const hipacc_view* img, int32_t num_bins, float scale,
+                         uint32_t* bins_dev, void* stream);
+
+/* ---- bilateral filter ------------------------------------------------- */
+/* The Bilateral_Filter sample's kernel (samples-public/3_Preprocessing/
+ * Bilateral_Filter/src/main.cpp:63-77) over a ksize x ksize domain:
+ *     c_r = 0.5f / (sigma_r*sigma_r); center = in(p)
+ *     s_t = expf(-c_r * (in(p+t) - center)^2) * spatial[t]
+ *     out(p) = cvt( (SUM s_t * in(p+t)) / (SUM s_t) )
+ * spatial: host ksize*ksize table, or NULL for
+ * spatial[t] = (float)exp(-(dx^2+dy^2) / (2*sigma_d^2)) (double exp).
+ * F32 in / F32 out, or U8 in / U8 out (cvt ROUND as in the sample).        */
+typedef struct hipacc_bilateral_desc {
+  hipacc_view in;
+  hipacc_view out;
+  int32_t ksize;             /* odd, 1..HIPACC_MAX_MASK_DIM                  */
+  float   sigma_d;
+  float   sigma_r;
+  const float* spatial;
+  int32_t boundary;
+  double  const_val;
+  int32_t cvt;
+} hipacc_bilateral_desc;
+int hipacc_hip_bilateral(const hipacc_bilateral_desc* desc, void* stream);
+
+/* ---- library / memory ------------------------------------------------- */
+/* hipaccInitCUDA (runtime/hipacc_cu_standalone.hpp:112-160) analogue:
+ * selects `device`, checks it is gfx950, creates the per-device state.     */
+int hipacc_hip_init(int device);
+const char* hipacc_hip_strerror(int status);
+const char* hipacc_hip_last_error(void);
+/* hipaccCreateMemory (runtime/hipacc_cu.tpp:33-83): pitched allocation, row
+ * pitch rounded up to `alignment` bytes (0 = 256).                          */
+int hipacc_hip_alloc2d(void** ptr, int32_t* stride_elems, int32_t width,
+                       int32_t height, int32_t elem_bytes, int32_t alignment);
+int hipacc_hip_free(void* ptr);
+/* hipaccWriteMemory / hipaccReadMemory (runtime/hipacc_cu.tpp:85-193):
+ * host rows are dense (width elements); blocking.                           */
+int hipacc_hip_upload2d(void* dst_dev, int32_t dst_stride_elems,
+                        const void* src_host, int32_t width, int32_t height,
+                        int32_t elem_bytes, void* stream);
+int hipacc_hip_download2d(void* dst_host, const void* src_dev,
+                          int32_t src_stride_elems, int32_t width,
+                          int32_t height, int32_t elem_bytes, void* stream);
+/* hipaccCopyMemoryRegion (runtime/hipacc_cu_standalone.hpp:166-216):
+ * device-to-device copy of a window, asynchronous.                         */
+int hipacc_hip_copy_region(const hipacc_view* src, const hipacc_view* dst,
+                           void* stream);
+/* Fills `count` elements of elem_bytes (1, 2 or 4) at dst_dev with the low
+ * bytes of `value`, with a kernel (safe inside stream capture).            */
+int hipacc_hip_fill(void* dst_dev, uint32_t value, size_t count,
+                    int32_t elem_bytes, void* stream);
+/* SplitMix64 synthetic pixels, counter form: element i of the image holds
+ * the (i+1)-th output of SplitMix64 seeded with `seed` (dense row-major
+ * numbering over width*height, written at the view's pitch).  U8: x >> 56;
+ * F32: (x >> 40) * 2^-24.  Used by tests and the benchmark.                */
+int hipacc_hip_fill_random(const hipacc_view* img, uint64_t seed, void* stream);
+
+/* hipacc_last_kernel_timing (runtime/hipacc_base.hpp:54-66): when timing is
+ * enabled, each entry point brackets its launches with hipEvents on the
+ * stream and synchronises, and the elapsed milliseconds are kept here.     */
+int   hipacc_hip_set_timing(int enabled);
+float hipacc_hip_last_kernel_timing(void);
+
+/* Library version string, e.g. "hipacc-gfx950 0.1". */
+const char* hipacc_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HIPACC_HIP_H */
