@@ -299,6 +299,27 @@ int hipacc_hip_local(const hipacc_local_desc* d, void* stream) {
   a.out_vec_ok = rows_aligned(d->out, std::min(16, vec * ebo));
   hipStream_t s = static_cast<hipStream_t>(stream);
   Timer tm(s);
+  // An integer SUM over a full all-ones window (a box sum) is the outer
+  // product of two all-ones rows; integer sums are exact in any order, so
+  // the separable running-sum kernel computes the same values.  CONSTANT
+  // qualifies only with the constant 0 (otherwise the border rows differ).
+  bool box = !accf && a.reduce == HIPACC_REDUCE_SUM && !a.wdev && d->in.width == d->out.width &&
+             d->in.height == d->out.height && a.mask_w <= 2 * HIPACC_MAX_SEP_RADIUS + 1 &&
+             (a.g.boundary != HIPACC_BOUNDARY_CONSTANT || a.g.const_i == 0);
+  for (int t = 0; box && t < n; ++t) box = a.domain[t] && (!a.use_weights || a.w.i[t] == 1);
+  if (box) {
+    SepArgs sa;
+    std::memset(&sa, 0, sizeof(sa));
+    sa.g = a.g;
+    sa.kx = a.mask_w;
+    sa.ky = a.mask_h;
+    for (int i = 0; i < sa.kx; ++i) sa.rowi[i] = 1;
+    for (int i = 0; i < sa.ky; ++i) sa.coli[i] = 1;
+    sa.box = 1;
+    sa.in_vec_ok = a.in_vec_ok;
+    sa.out_vec_ok = a.out_vec_ok;
+    return tm.finish(launch_separable(sa, 0, 0, s, 0));
+  }
   return tm.finish(launch_local(a, accf, s, 0));
 }
 

@@ -51,16 +51,31 @@ def test_separable_u8_gaussian13(gpu, mode):
 @pytest.mark.parametrize("k", [3, 5, 7])
 def test_box_sum_separable_equals_2d_reference(gpu, mode, k):
     """uint8 -> uint32 box sum: the separable integer path against the 2D
-    Domain reduce of the oracle (exact: integer sums are associative)."""
+    Domain reduce of the oracle (exact: integer sums are associative).  With
+    CONSTANT the two agree for the constant 0; for another constant the
+    two-pass program differs by definition (its intermediate image's rows
+    outside the window read the constant itself) and is checked against the
+    two-pass oracle instead."""
     print(f"box k={k} mode={mode}", flush=True)
     src = ob.random_image((70, 1300), np.uint8, 5 + k)
-    ref = np.zeros(src.shape, np.uint32)
-    ob.local(src, ref, k, k, acc_float=False, boundary=mode, const_val=250)
-    dst = empty_gpu(src.shape, np.uint32)
     ones = np.ones(k, np.int32)
-    op.separable(to_gpu(src), dst, ones, ones, acc_float=False, boundary=mode, const_val=250,
+    for cval in (0, 250):
+        ref = np.zeros(src.shape, np.uint32)
+        if mode == capi.CONSTANT and cval:
+            ob.separable(src, ref, ones, ones, acc_float=False, boundary=mode, const_val=cval)
+        else:
+            ob.local(src, ref, k, k, acc_float=False, boundary=mode, const_val=cval)
+        dst = empty_gpu(src.shape, np.uint32)
+        op.separable(to_gpu(src), dst, ones, ones, acc_float=False, boundary=mode, const_val=cval,
+                     dst_dtype=capi.U32)
+        np.testing.assert_array_equal(to_host(dst, np.uint32), ref)
+        # the 2D entry point with an all-ones integer domain
+        dst2 = empty_gpu(src.shape, np.uint32)
+        op.local(to_gpu(src), dst2, k, k, acc_float=False, boundary=mode, const_val=cval,
                  dst_dtype=capi.U32)
-    np.testing.assert_array_equal(to_host(dst, np.uint32), ref)
+        ref2 = np.zeros(src.shape, np.uint32)
+        ob.local(src, ref2, k, k, acc_float=False, boundary=mode, const_val=cval)
+        np.testing.assert_array_equal(to_host(dst2, np.uint32), ref2)
 
 
 def test_box_7x7_golden(gpu):
