@@ -63,6 +63,16 @@ class BilateralDesc(ctypes.Structure):
                 ("const_val", c_double), ("cvt", c_int32)]
 
 
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("kx", c_int32), ("ky", c_int32), ("in_dtype", c_int32),
+                ("out_dtype", c_int32), ("acc_dtype", c_int32), ("boundary", c_int32),
+                ("fp_contract", c_int32), ("weights_dev", c_int32), ("variant", c_int32),
+                ("param", c_int32), ("ms", c_float)]
+
+
+VAR_AUTO, VAR_TILE, VAR_STREAM, VAR_GENERIC = range(4)
+VARIANT_NAMES = {VAR_TILE: "tile", VAR_STREAM: "stream", VAR_GENERIC: "generic"}
+
 # Every exported entry point with its ctypes signature; tests check that the
 # library exports exactly these names.
 SIGNATURES = {
@@ -90,6 +100,14 @@ SIGNATURES = {
     "hipacc_hip_fill_random": (c_int32, [POINTER(View), c_uint64, c_void_p]),
     "hipacc_hip_set_timing": (c_int32, [c_int32]),
     "hipacc_hip_last_kernel_timing": (c_float, []),
+    "hipacc_hip_local_variant": (c_int32, [POINTER(LocalDesc), c_int32, c_int32, c_void_p]),
+    "hipacc_hip_separable_variant": (c_int32, [POINTER(SeparableDesc), c_int32, c_int32, c_void_p]),
+    "hipacc_hip_tune_local": (c_int32, [POINTER(LocalDesc), c_int32, POINTER(TuneEntry), c_void_p]),
+    "hipacc_hip_tune_separable": (c_int32, [POINTER(SeparableDesc), c_int32, POINTER(TuneEntry),
+                                            c_void_p]),
+    "hipacc_hip_tune_set": (c_int32, [POINTER(TuneEntry)]),
+    "hipacc_hip_tune_save": (c_int32, [ctypes.c_char_p]),
+    "hipacc_hip_tune_load": (c_int32, [ctypes.c_char_p]),
 }
 
 _LIB = None

@@ -244,9 +244,14 @@ int hipacc_hip_set_timing(int enabled) {
 }
 float hipacc_hip_last_kernel_timing(void) { return g_last_ms; }
 
-int hipacc_hip_local(const hipacc_local_desc* d, void* stream) {
+}  // extern "C"
+
+namespace hh {
+
+// Translates a local descriptor.  *box is set when the operator is a box sum
+// that the separable running-sum kernel computes exactly (then sa is set).
+static int prep_local(const hipacc_local_desc* d, LocalArgs& a, int& accf, bool& box, SepArgs& sa) {
   if (!d) return set_error(HIPACC_ERR_INVALID_ARG, "local: desc is NULL");
-  LocalArgs a;
   std::memset(&a, 0, sizeof(a));
   int st = make_geom(a.g, d->in, d->out, d->boundary, d->const_val, d->cvt, d->post_div);
   if (st) return st;
@@ -259,7 +264,7 @@ int hipacc_hip_local(const hipacc_local_desc* d, void* stream) {
   if (d->reduce < HIPACC_REDUCE_SUM || d->reduce > HIPACC_REDUCE_PROD)
     return set_error(HIPACC_ERR_INVALID_ARG, "local: reduce %d", d->reduce);
   const int n = d->mask_w * d->mask_h;
-  const bool accf = d->acc_dtype == HIPACC_F32;
+  accf = d->acc_dtype == HIPACC_F32;
   if (d->use_weights) {
     if (d->weights_dev) {
       if (!accf) return set_error(HIPACC_ERR_INVALID_ARG, "local: weights_dev needs acc_dtype F32");
@@ -297,18 +302,15 @@ int hipacc_hip_local(const hipacc_local_desc* d, void* stream) {
   const int vec = 16 / ebi;
   a.in_vec_ok = rows_aligned(d->in, 16) && (d->in.offset_x % vec) == 0;
   a.out_vec_ok = rows_aligned(d->out, std::min(16, vec * ebo));
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  Timer tm(s);
   // An integer SUM over a full all-ones window (a box sum) is the outer
   // product of two all-ones rows; integer sums are exact in any order, so
-  // the separable running-sum kernel computes the same values.  CONSTANT
+  // the separable running-sum kernels compute the same values.  CONSTANT
   // qualifies only with the constant 0 (otherwise the border rows differ).
-  bool box = !accf && a.reduce == HIPACC_REDUCE_SUM && !a.wdev && d->in.width == d->out.width &&
-             d->in.height == d->out.height && a.mask_w <= 2 * HIPACC_MAX_SEP_RADIUS + 1 &&
-             (a.g.boundary != HIPACC_BOUNDARY_CONSTANT || a.g.const_i == 0);
+  box = !accf && a.reduce == HIPACC_REDUCE_SUM && !a.wdev && d->in.width == d->out.width &&
+        d->in.height == d->out.height && a.mask_w <= 2 * HIPACC_MAX_SEP_RADIUS + 1 &&
+        (a.g.boundary != HIPACC_BOUNDARY_CONSTANT || a.g.const_i == 0);
   for (int t = 0; box && t < n; ++t) box = a.domain[t] && (!a.use_weights || a.w.i[t] == 1);
   if (box) {
-    SepArgs sa;
     std::memset(&sa, 0, sizeof(sa));
     sa.g = a.g;
     sa.kx = a.mask_w;
@@ -318,14 +320,12 @@ int hipacc_hip_local(const hipacc_local_desc* d, void* stream) {
     sa.box = 1;
     sa.in_vec_ok = a.in_vec_ok;
     sa.out_vec_ok = a.out_vec_ok;
-    return tm.finish(launch_separable(sa, 0, 0, s, 0));
   }
-  return tm.finish(launch_local(a, accf, s, 0));
+  return HIPACC_OK;
 }
 
-int hipacc_hip_separable(const hipacc_separable_desc* d, void* stream) {
+static int prep_separable(const hipacc_separable_desc* d, SepArgs& a, int& accf) {
   if (!d) return set_error(HIPACC_ERR_INVALID_ARG, "separable: desc is NULL");
-  SepArgs a;
   std::memset(&a, 0, sizeof(a));
   int st = make_geom(a.g, d->in, d->out, d->boundary, d->const_val, d->cvt, d->post_div);
   if (st) return st;
@@ -336,7 +336,7 @@ int hipacc_hip_separable(const hipacc_separable_desc* d, void* stream) {
   if (d->in.width != d->out.width || d->in.height != d->out.height)
     return set_error(HIPACC_ERR_UNSUPPORTED,
                      "separable: iteration space must have the accessor window's size");
-  const bool accf = d->acc_dtype == HIPACC_F32;
+  accf = d->acc_dtype == HIPACC_F32;
   if (!accf && d->acc_dtype != HIPACC_I32)
     return set_error(HIPACC_ERR_INVALID_ARG, "separable: acc_dtype must be F32 or I32");
   a.kx = d->ksize_x;
@@ -359,9 +359,167 @@ int hipacc_hip_separable(const hipacc_separable_desc* d, void* stream) {
   const int vec = 16 / ebi;
   a.in_vec_ok = rows_aligned(d->in, 16) && (d->in.offset_x % vec) == 0;
   a.out_vec_ok = rows_aligned(d->out, std::min(16, vec * ebo));
-  hipStream_t s = static_cast<hipStream_t>(stream);
+  return HIPACC_OK;
+}
+
+static int run_local(const hipacc_local_desc* d, int variant, int param, hipStream_t s) {
+  LocalArgs a;
+  SepArgs sa;
+  int accf = 0;
+  bool box = false;
+  int st = prep_local(d, a, accf, box, sa);
+  if (st) return st;
   Timer tm(s);
-  return tm.finish(launch_separable(a, accf, d->fp_contract != 0, s, 0));
+  if (box) return tm.finish(launch_separable(sa, 0, 0, s, variant, param));
+  return tm.finish(launch_local(a, accf, s, variant, param));
+}
+
+static int run_separable(const hipacc_separable_desc* d, int variant, int param, hipStream_t s) {
+  SepArgs a;
+  int accf = 0;
+  int st = prep_separable(d, a, accf);
+  if (st) return st;
+  Timer tm(s);
+  return tm.finish(launch_separable(a, accf, d->fp_contract != 0, s, variant, param));
+}
+
+// Times every variant/parameter candidate of one operator on the caller's
+// buffers and records the fastest in the tune cache.
+template <typename RunFn>
+static int tune_candidates(RunFn run, const TuneKey& key, int iters, hipStream_t s,
+                           hipacc_tune_entry* best) {
+  static const int cands[][2] = {{kVarStream, 8},  {kVarStream, 16}, {kVarStream, 32},
+                                 {kVarStream, 64}, {kVarStream, 128}, {kVarTile, 2},
+                                 {kVarTile, 4},    {kVarTile, 8},     {kVarGeneric, 0}};
+  if (iters < 1) iters = 1;
+  hipEvent_t e0, e1;
+  hipError_t e = hipEventCreate(&e0);
+  if (e != hipSuccess) return hip_status(e, "hipEventCreate");
+  if ((e = hipEventCreate(&e1)) != hipSuccess) {
+    hipEventDestroy(e0);
+    return hip_status(e, "hipEventCreate");
+  }
+  TuneVal bestv{0, 0, 0.0f};
+  int st = HIPACC_OK;
+  for (const auto& c : cands) {
+    st = run(c[0], c[1]);
+    if (st == HIPACC_ERR_UNSUPPORTED) {
+      st = HIPACC_OK;
+      continue;
+    }
+    if (st) break;
+    hipEventRecord(e0, s);
+    for (int i = 0; i < iters && st == HIPACC_OK; ++i) st = run(c[0], c[1]);
+    hipEventRecord(e1, s);
+    if (st) break;
+    if ((e = hipEventSynchronize(e1)) != hipSuccess) {
+      st = hip_status(e, "hipEventSynchronize");
+      break;
+    }
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, e0, e1);
+    ms /= iters;
+    if (bestv.variant == 0 || ms < bestv.ms) bestv = TuneVal{c[0], c[1], ms};
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (st) return st;
+  if (bestv.variant == 0) return set_error(HIPACC_ERR_UNSUPPORTED, "tune: no candidate ran");
+  tune_store(key, bestv);
+  if (best) {
+    best->kind = key.kind;
+    best->kx = key.kx;
+    best->ky = key.ky;
+    best->in_dtype = key.in_dt;
+    best->out_dtype = key.out_dt;
+    best->acc_dtype = key.acc_f ? HIPACC_F32 : HIPACC_I32;
+    best->boundary = key.boundary;
+    best->fp_contract = key.fma;
+    best->weights_dev = key.wdev;
+    best->variant = bestv.variant;
+    best->param = bestv.param;
+    best->ms = bestv.ms;
+  }
+  return HIPACC_OK;
+}
+
+}  // namespace hh
+
+extern "C" {
+
+int hipacc_hip_local(const hipacc_local_desc* d, void* stream) {
+  return run_local(d, kVarAuto, 0, static_cast<hipStream_t>(stream));
+}
+
+int hipacc_hip_local_variant(const hipacc_local_desc* d, int32_t variant, int32_t param, void* stream) {
+  if (variant < kVarAuto || variant > kVarGeneric)
+    return set_error(HIPACC_ERR_INVALID_ARG, "local_variant: variant %d", variant);
+  return run_local(d, variant, param, static_cast<hipStream_t>(stream));
+}
+
+int hipacc_hip_separable(const hipacc_separable_desc* d, void* stream) {
+  return run_separable(d, kVarAuto, 0, static_cast<hipStream_t>(stream));
+}
+
+int hipacc_hip_separable_variant(const hipacc_separable_desc* d, int32_t variant, int32_t param,
+                                 void* stream) {
+  if (variant < kVarAuto || variant > kVarGeneric)
+    return set_error(HIPACC_ERR_INVALID_ARG, "separable_variant: variant %d", variant);
+  return run_separable(d, variant, param, static_cast<hipStream_t>(stream));
+}
+
+int hipacc_hip_tune_local(const hipacc_local_desc* d, int32_t iters, hipacc_tune_entry* best,
+                          void* stream) {
+  LocalArgs a;
+  SepArgs sa;
+  int accf = 0;
+  bool box = false;
+  int st = prep_local(d, a, accf, box, sa);
+  if (st) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (box) {
+    TuneKey k{kTuneBox, sa.kx, sa.ky, sa.g.in_dtype, sa.g.out_dtype, 0, sa.g.boundary, 0, 0};
+    return tune_candidates([&](int v, int p) { return launch_separable(sa, 0, 0, s, v, p); }, k,
+                           iters, s, best);
+  }
+  TuneKey k{kTuneLocal, a.mask_w, a.mask_h, a.g.in_dtype, a.g.out_dtype, accf, a.g.boundary,
+            a.fma, a.wdev != nullptr};
+  return tune_candidates([&](int v, int p) { return launch_local(a, accf, s, v, p); }, k, iters, s,
+                         best);
+}
+
+int hipacc_hip_tune_separable(const hipacc_separable_desc* d, int32_t iters, hipacc_tune_entry* best,
+                              void* stream) {
+  SepArgs a;
+  int accf = 0;
+  int st = prep_separable(d, a, accf);
+  if (st) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int fma = d->fp_contract != 0;
+  TuneKey k{a.box ? kTuneBox : kTuneSeparable, a.kx, a.ky, a.g.in_dtype, a.g.out_dtype, accf,
+            a.g.boundary, fma, 0};
+  return tune_candidates([&](int v, int p) { return launch_separable(a, accf, fma, s, v, p); }, k,
+                         iters, s, best);
+}
+
+int hipacc_hip_tune_set(const hipacc_tune_entry* e) {
+  if (!e) return set_error(HIPACC_ERR_INVALID_ARG, "tune_set: NULL entry");
+  if (e->variant < kVarTile || e->variant > kVarGeneric)
+    return set_error(HIPACC_ERR_INVALID_ARG, "tune_set: variant %d", e->variant);
+  TuneKey k{e->kind, e->kx, e->ky, e->in_dtype, e->out_dtype, e->acc_dtype == HIPACC_F32,
+            e->boundary, e->fp_contract, e->weights_dev};
+  tune_store(k, TuneVal{e->variant, e->param, e->ms});
+  return HIPACC_OK;
+}
+
+int hipacc_hip_tune_save(const char* path) {
+  if (!path) return set_error(HIPACC_ERR_INVALID_ARG, "tune_save: NULL path");
+  return tune_save(path);
+}
+
+int hipacc_hip_tune_load(const char* path) {
+  if (!path) return set_error(HIPACC_ERR_INVALID_ARG, "tune_load: NULL path");
+  return tune_load(path);
 }
 
 int hipacc_hip_point(const hipacc_point_desc* d, void* stream) {

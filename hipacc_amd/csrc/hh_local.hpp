@@ -46,8 +46,29 @@ struct LocalArgs {
   Weights w;
 };
 
-// Launch one local operator; returns a HIPACC_* status.
-int launch_local(const LocalArgs& a, int acc_is_float, hipStream_t stream, int ppt_hint);
+// Kernel variants of a local operator (the autotuner's choice):
+//  tile    LDS-staged tile + halo, PPT rows per wave (hh_local_fast.hpp)
+//  stream  row-streaming strips, `param` rows per segment (hh_stream.hpp)
+//  generic one pixel per thread, any shape / mode (local.hip)
+enum Variant { kVarAuto = 0, kVarTile = 1, kVarStream = 2, kVarGeneric = 3 };
+
+// Operator kinds the tuner keys on.
+enum TuneKind { kTuneLocal = 0, kTuneSeparable = 1, kTuneBox = 2 };
+struct TuneKey {
+  int kind, kx, ky, in_dt, out_dt, acc_f, boundary, fma, wdev;
+};
+struct TuneVal {
+  int variant, param;
+  float ms;
+};
+bool tune_lookup(const TuneKey& k, TuneVal& v);
+void tune_store(const TuneKey& k, const TuneVal& v);
+int tune_save(const char* path);
+int tune_load(const char* path);
+
+// Launch one local operator; returns a HIPACC_* status.  variant/param
+// force a kernel variant (kVarAuto: the tuned or default choice).
+int launch_local(const LocalArgs& a, int acc_is_float, hipStream_t stream, int variant, int param);
 // Fast weighted-SUM instantiations, one table per pixel-type family; each
 // returns HIPACC_ERR_UNSUPPORTED when it has no matching instantiation.
 int dispatch_sum_f32(const LocalArgs& a, int ppt, hipStream_t s);
@@ -65,7 +86,9 @@ struct SepArgs {
   int box;          // integer weights are all 1: running sums are exact
 };
 int launch_separable(const SepArgs& a, int acc_is_float, int fp_contract,
-                     hipStream_t stream, int ppt_hint);
+                     hipStream_t stream, int variant, int param);
+int dispatch_stream_local(const LocalArgs& a, int acc_is_float, int seg, hipStream_t s);
+int dispatch_stream_sep(const SepArgs& a, int acc_is_float, int fma, int seg, hipStream_t s);
 
 // ---- point operators ------------------------------------------------------
 struct PointArgs {

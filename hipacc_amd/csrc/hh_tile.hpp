@@ -74,13 +74,28 @@ __device__ __forceinline__ void stage_tile(TI* __restrict__ tile, const TI* __re
   raw_region(g, lx, hx, ly, hy);
   const bool interior = vec_ok && ix0 >= lx && ix0 + LW <= hx && iy0 >= ly && iy0 + LH <= hy;
   if (interior) {
+    // Every load of the thread is issued before the first LDS write, so all
+    // of them are in flight together.
+    constexpr int NL = (LH * NV + 255) / 256;
     const TI* src = in + (long long)iy0 * g.in_stride + ix0;
-#pragma unroll 4
-    for (int i = tid; i < LH * NV; i += 256) {
-      const int r = i / NV;
-      const int c = i - r * NV;
-      const uint4 w = *reinterpret_cast<const uint4*>(src + (long long)r * g.in_stride + c * VEC);
-      *reinterpret_cast<uint4*>(tile + r * LW + c * VEC) = w;
+    uint4 w[NL];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+      const int i = tid + k * 256;
+      if (i < LH * NV) {
+        const int r = i / NV;
+        const int c = i - r * NV;
+        w[k] = *reinterpret_cast<const uint4*>(src + (long long)r * g.in_stride + c * VEC);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+      const int i = tid + k * 256;
+      if (i < LH * NV) {
+        const int r = i / NV;
+        const int c = i - r * NV;
+        *reinterpret_cast<uint4*>(tile + r * LW + c * VEC) = w[k];
+      }
     }
   } else {
     const bool cmode = g.boundary == HIPACC_BOUNDARY_CONSTANT;

@@ -89,8 +89,35 @@ __global__ __launch_bounds__(256) void local_generic_kernel(const LocalArgs a) {
     store_i(out, g.out_dtype, oidx, acc, g.cvt, g.post_div);
 }
 
-int launch_local(const LocalArgs& a, int acc_is_float, hipStream_t s, int ppt_hint) {
-  // The fast kernels implement SUM with weights (or a full all-ones domain);
+static int launch_generic(const LocalArgs& a, int acc_is_float, hipStream_t s) {
+  dim3 grid(ceil_div(a.g.is_w, 64), ceil_div(a.g.is_h, 4), a.g.batch);
+  if (acc_is_float)
+    hipLaunchKernelGGL(local_generic_kernel<float>, grid, dim3(64, 4), 0, s, a);
+  else
+    hipLaunchKernelGGL(local_generic_kernel<int32_t>, grid, dim3(64, 4), 0, s, a);
+  return hip_status(hipGetLastError(), "local_generic_kernel launch");
+}
+
+static int launch_local_variant(const LocalArgs& f, int acc_is_float, hipStream_t s, int variant,
+                                int param) {
+  switch (variant) {
+    case kVarStream:
+      return dispatch_stream_local(f, acc_is_float, param, s);
+    case kVarTile: {
+      if (!acc_is_float) return dispatch_sum_int(f, param, s);
+      const int st = dispatch_sum_f32(f, param, s);
+      return st == HIPACC_ERR_UNSUPPORTED ? dispatch_sum_u8(f, param, s) : st;
+    }
+    default:
+      return HIPACC_ERR_UNSUPPORTED;
+  }
+}
+
+// variant kVarAuto: the tuned choice if any, else stream -> tile -> generic.
+// A forced variant returns HIPACC_ERR_UNSUPPORTED when it has no kernel for
+// the operator's shape.
+int launch_local(const LocalArgs& a, int acc_is_float, hipStream_t s, int variant, int param) {
+  // The tile and stream kernels implement SUM with weights (or a full domain);
   // domain holes become zero weights, which leaves every finite sum unchanged.
   bool fast = a.reduce == HIPACC_REDUCE_SUM && a.mask_w == a.mask_h;
   if (fast && a.wdev) {
@@ -98,34 +125,33 @@ int launch_local(const LocalArgs& a, int acc_is_float, hipStream_t s, int ppt_hi
       if (!a.domain[t]) fast = false;
     if (!a.use_weights) fast = false;
   }
-  if (fast) {
-    LocalArgs f = a;
-    const int n = a.mask_w * a.mask_h;
-    for (int t = 0; t < n; ++t) {
-      if (acc_is_float) {
-        float w = a.use_weights ? a.w.f[t] : 1.0f;
-        f.w.f[t] = a.domain[t] ? w : 0.0f;
-      } else {
-        int32_t w = a.use_weights ? a.w.i[t] : 1;
-        f.w.i[t] = a.domain[t] ? w : 0;
-      }
-    }
-    f.use_weights = 1;
-    int st = HIPACC_ERR_UNSUPPORTED;
+  if (variant == kVarGeneric) return launch_generic(a, acc_is_float, s);
+  if (!fast) return variant == kVarAuto ? launch_generic(a, acc_is_float, s) : HIPACC_ERR_UNSUPPORTED;
+  LocalArgs f = a;
+  const int n = a.mask_w * a.mask_h;
+  for (int t = 0; t < n; ++t) {
     if (acc_is_float) {
-      st = dispatch_sum_f32(f, ppt_hint, s);
-      if (st == HIPACC_ERR_UNSUPPORTED) st = dispatch_sum_u8(f, ppt_hint, s);
+      float w = a.use_weights ? a.w.f[t] : 1.0f;
+      f.w.f[t] = a.domain[t] ? w : 0.0f;
     } else {
-      st = dispatch_sum_int(f, ppt_hint, s);
+      int32_t w = a.use_weights ? a.w.i[t] : 1;
+      f.w.i[t] = a.domain[t] ? w : 0;
     }
+  }
+  f.use_weights = 1;
+  if (variant != kVarAuto) return launch_local_variant(f, acc_is_float, s, variant, param);
+  TuneKey k{kTuneLocal, a.mask_w, a.mask_h, a.g.in_dtype, a.g.out_dtype, acc_is_float,
+            a.g.boundary, a.fma, a.wdev != nullptr};
+  TuneVal v;
+  if (tune_lookup(k, v)) {
+    if (v.variant == kVarGeneric) return launch_generic(a, acc_is_float, s);
+    const int st = launch_local_variant(f, acc_is_float, s, v.variant, v.param);
     if (st != HIPACC_ERR_UNSUPPORTED) return st;
   }
-  dim3 grid(ceil_div(a.g.is_w, 64), ceil_div(a.g.is_h, 4), a.g.batch);
-  if (acc_is_float)
-    hipLaunchKernelGGL(local_generic_kernel<float>, grid, dim3(64, 4), 0, s, a);
-  else
-    hipLaunchKernelGGL(local_generic_kernel<int32_t>, grid, dim3(64, 4), 0, s, a);
-  return hip_status(hipGetLastError(), "local_generic_kernel launch");
+  int st = launch_local_variant(f, acc_is_float, s, kVarStream, 0);
+  if (st == HIPACC_ERR_UNSUPPORTED) st = launch_local_variant(f, acc_is_float, s, kVarTile, 0);
+  if (st == HIPACC_ERR_UNSUPPORTED) st = launch_generic(a, acc_is_float, s);
+  return st;
 }
 
 }  // namespace hh

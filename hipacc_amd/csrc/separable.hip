@@ -196,10 +196,7 @@ __global__ __launch_bounds__(256) void separable_generic_kernel(const SepArgs a)
   else store_i(out, g.out_dtype, oidx, acc, g.cvt, g.post_div);
 }
 
-int launch_separable(const SepArgs& a, int acc_is_float, int fp_contract, hipStream_t s,
-                     int ppt_hint) {
-  int st = dispatch_sep(a, acc_is_float, fp_contract, ppt_hint, s);
-  if (st != HIPACC_ERR_UNSUPPORTED) return st;
+static int launch_sep_generic(const SepArgs& a, int acc_is_float, int fp_contract, hipStream_t s) {
   // The generic kernel reuses `box` as the float contraction flag.
   SepArgs b = a;
   if (acc_is_float) b.box = fp_contract;
@@ -209,6 +206,33 @@ int launch_separable(const SepArgs& a, int acc_is_float, int fp_contract, hipStr
   else
     hipLaunchKernelGGL(separable_generic_kernel<int32_t>, grid, dim3(64, 4), 0, s, b);
   return hip_status(hipGetLastError(), "separable_generic_kernel launch");
+}
+
+static int launch_sep_variant(const SepArgs& a, int acc_is_float, int fma, hipStream_t s,
+                              int variant, int param) {
+  switch (variant) {
+    case kVarStream: return dispatch_stream_sep(a, acc_is_float, fma, param, s);
+    case kVarTile: return dispatch_sep(a, acc_is_float, fma, param, s);
+    case kVarGeneric: return launch_sep_generic(a, acc_is_float, fma, s);
+  }
+  return HIPACC_ERR_UNSUPPORTED;
+}
+
+// variant kVarAuto: the tuned choice if any, else stream -> tile -> generic.
+int launch_separable(const SepArgs& a, int acc_is_float, int fp_contract, hipStream_t s,
+                     int variant, int param) {
+  if (variant != kVarAuto) return launch_sep_variant(a, acc_is_float, fp_contract, s, variant, param);
+  TuneKey k{a.box ? kTuneBox : kTuneSeparable, a.kx, a.ky, a.g.in_dtype, a.g.out_dtype,
+            acc_is_float, a.g.boundary, fp_contract, 0};
+  TuneVal v;
+  if (tune_lookup(k, v)) {
+    const int st = launch_sep_variant(a, acc_is_float, fp_contract, s, v.variant, v.param);
+    if (st != HIPACC_ERR_UNSUPPORTED) return st;
+  }
+  int st = launch_sep_variant(a, acc_is_float, fp_contract, s, kVarStream, 0);
+  if (st == HIPACC_ERR_UNSUPPORTED) st = launch_sep_variant(a, acc_is_float, fp_contract, s, kVarTile, 0);
+  if (st == HIPACC_ERR_UNSUPPORTED) st = launch_sep_generic(a, acc_is_float, fp_contract, s);
+  return st;
 }
 
 }  // namespace hh

@@ -64,19 +64,42 @@ def _stream() -> int:
 
 class Call:
     """A prepared C-ABI call: the descriptor is built once (host tables kept
-    alive here) and ``launch(stream)`` only crosses the ABI."""
+    alive here) and ``launch(stream)`` only crosses the ABI.  ``variant``
+    forces a kernel variant (capi.VAR_*) with its tile parameter."""
 
-    def __init__(self, fn_name: str, desc, keep=()):
-        self.fn = getattr(capi.lib(), fn_name)
+    _VARIANT_FN = {"hipacc_hip_local": "hipacc_hip_local_variant",
+                   "hipacc_hip_separable": "hipacc_hip_separable_variant"}
+    _TUNE_FN = {"hipacc_hip_local": "hipacc_hip_tune_local",
+                "hipacc_hip_separable": "hipacc_hip_tune_separable"}
+
+    def __init__(self, fn_name: str, desc, keep=(), variant: int = 0, param: int = 0):
         self.name = fn_name
         self.desc = desc
         self.ref = ctypes.byref(desc)
         self.keep = list(keep)
+        self.variant, self.param = variant, param
+        if variant:
+            self.fn = getattr(capi.lib(), self._VARIANT_FN[fn_name])
+            self.extra = (variant, param)
+        else:
+            self.fn = getattr(capi.lib(), fn_name)
+            self.extra = ()
 
     def launch(self, stream: Optional[int] = None) -> None:
-        st = self.fn(self.ref, _stream() if stream is None else stream)
+        st = self.fn(self.ref, *self.extra, _stream() if stream is None else stream)
         if st:
             capi.check(st, self.name)
+
+    def tune(self, iters: int = 20) -> capi.TuneEntry:
+        """Times every kernel variant of this operator on its buffers and
+        records the fastest in the library's tune cache."""
+        e = capi.TuneEntry()
+        fn = getattr(capi.lib(), self._TUNE_FN[self.name])
+        capi.check(fn(self.ref, iters, ctypes.byref(e), _stream()), self._TUNE_FN[self.name])
+        return e
+
+    def with_variant(self, variant: int, param: int = 0) -> "Call":
+        return Call(self.name, self.desc, self.keep, variant, param)
 
 
 def _arr(values, np_dtype):

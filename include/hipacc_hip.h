@@ -260,6 +260,43 @@ typedef struct hipacc_bilateral_desc {
 } hipacc_bilateral_desc;
 int hipacc_hip_bilateral(const hipacc_bilateral_desc* desc, void* stream);
 
+/* ---- kernel variants and the autotuner ------------------------------- */
+/* A local or separable operator runs one of three kernel variants:
+ *   1 tile    LDS-staged tile + halo; param = output rows per wave (2/4/8)
+ *   2 stream  row-streaming strips with a register prefetch ring and
+ *             cross-lane halos; param = output rows per wave segment
+ *   3 generic one pixel per thread, any shape and reduction mode
+ * All variants compute identical results.  The plain entry points use the
+ * tuned variant of the operator's shape when the tune cache has one, else
+ * stream, else tile, else generic.  The tuner times every candidate on the
+ * caller's buffers (the output is overwritten) and records the fastest per
+ * (kind, mask size, pixel types, accumulator, boundary, contraction, device
+ * weights).  The cache lives in the process and can be saved / loaded.
+ * Replaces the reference's fixed per-device kernel configuration
+ * (lib/DSL/ClassRepresentation.cpp:256-489 setDefaultConfig, -use-config /
+ * -pixels-per-thread, include/hipacc/Device/TargetDescription.h:64-180).   */
+typedef struct hipacc_tune_entry {
+  int32_t kind;        /* 0 local weighted SUM, 1 separable, 2 box sum         */
+  int32_t kx, ky;
+  int32_t in_dtype, out_dtype, acc_dtype, boundary, fp_contract, weights_dev;
+  int32_t variant;     /* 1 tile, 2 stream, 3 generic                         */
+  int32_t param;
+  float   ms;          /* measured time per launch                            */
+} hipacc_tune_entry;
+/* variant 0 = automatic choice; a forced variant without a kernel for the
+ * operator's shape returns HIPACC_ERR_UNSUPPORTED.                           */
+int hipacc_hip_local_variant(const hipacc_local_desc* desc, int32_t variant, int32_t param,
+                             void* stream);
+int hipacc_hip_separable_variant(const hipacc_separable_desc* desc, int32_t variant,
+                                 int32_t param, void* stream);
+int hipacc_hip_tune_local(const hipacc_local_desc* desc, int32_t iters, hipacc_tune_entry* best,
+                          void* stream);
+int hipacc_hip_tune_separable(const hipacc_separable_desc* desc, int32_t iters,
+                              hipacc_tune_entry* best, void* stream);
+int hipacc_hip_tune_set(const hipacc_tune_entry* entry);
+int hipacc_hip_tune_save(const char* path);
+int hipacc_hip_tune_load(const char* path);
+
 /* ---- library / memory ------------------------------------------------- */
 /* hipaccInitCUDA (runtime/hipacc_cu_standalone.hpp:112-160) analogue:
  * selects `device`, checks it is gfx950, creates the per-device state.     */
