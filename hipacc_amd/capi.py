@@ -70,8 +70,8 @@ class TuneEntry(ctypes.Structure):
                 ("param", c_int32), ("ms", c_float)]
 
 
-VAR_AUTO, VAR_TILE, VAR_STREAM, VAR_GENERIC = range(4)
-VARIANT_NAMES = {VAR_TILE: "tile", VAR_STREAM: "stream", VAR_GENERIC: "generic"}
+VAR_AUTO, VAR_TILE, VAR_STREAM, VAR_GENERIC, VAR_WIDE = range(5)
+VARIANT_NAMES = {VAR_TILE: "tile", VAR_STREAM: "stream", VAR_GENERIC: "generic", VAR_WIDE: "wide"}
 
 # Every exported entry point with its ctypes signature; tests check that the
 # library exports exactly these names.

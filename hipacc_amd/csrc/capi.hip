@@ -388,9 +388,11 @@ static int run_separable(const hipacc_separable_desc* d, int variant, int param,
 template <typename RunFn>
 static int tune_candidates(RunFn run, const TuneKey& key, int iters, hipStream_t s,
                            hipacc_tune_entry* best) {
-  static const int cands[][2] = {{kVarStream, 8},  {kVarStream, 16}, {kVarStream, 32},
-                                 {kVarStream, 64}, {kVarStream, 128}, {kVarTile, 2},
-                                 {kVarTile, 4},    {kVarTile, 8},     {kVarGeneric, 0}};
+  static const int cands[][2] = {
+      {kVarStream, 8},      {kVarStream, 16},     {kVarStream, 24},     {kVarStream, 32},
+      {kVarStream, 64},     {kVarStreamWide, 8},  {kVarStreamWide, 16}, {kVarStreamWide, 24},
+      {kVarStreamWide, 32}, {kVarStreamWide, 64}, {kVarTile, 2},        {kVarTile, 4},
+      {kVarTile, 8},        {kVarGeneric, 0}};
   if (iters < 1) iters = 1;
   hipEvent_t e0, e1;
   hipError_t e = hipEventCreate(&e0);
@@ -452,7 +454,7 @@ int hipacc_hip_local(const hipacc_local_desc* d, void* stream) {
 }
 
 int hipacc_hip_local_variant(const hipacc_local_desc* d, int32_t variant, int32_t param, void* stream) {
-  if (variant < kVarAuto || variant > kVarGeneric)
+  if (variant < kVarAuto || variant > kVarStreamWide)
     return set_error(HIPACC_ERR_INVALID_ARG, "local_variant: variant %d", variant);
   return run_local(d, variant, param, static_cast<hipStream_t>(stream));
 }
@@ -463,7 +465,7 @@ int hipacc_hip_separable(const hipacc_separable_desc* d, void* stream) {
 
 int hipacc_hip_separable_variant(const hipacc_separable_desc* d, int32_t variant, int32_t param,
                                  void* stream) {
-  if (variant < kVarAuto || variant > kVarGeneric)
+  if (variant < kVarAuto || variant > kVarStreamWide)
     return set_error(HIPACC_ERR_INVALID_ARG, "separable_variant: variant %d", variant);
   return run_separable(d, variant, param, static_cast<hipStream_t>(stream));
 }
@@ -504,7 +506,7 @@ int hipacc_hip_tune_separable(const hipacc_separable_desc* d, int32_t iters, hip
 
 int hipacc_hip_tune_set(const hipacc_tune_entry* e) {
   if (!e) return set_error(HIPACC_ERR_INVALID_ARG, "tune_set: NULL entry");
-  if (e->variant < kVarTile || e->variant > kVarGeneric)
+  if (e->variant < kVarTile || e->variant > kVarStreamWide)
     return set_error(HIPACC_ERR_INVALID_ARG, "tune_set: variant %d", e->variant);
   TuneKey k{e->kind, e->kx, e->ky, e->in_dtype, e->out_dtype, e->acc_dtype == HIPACC_F32,
             e->boundary, e->fp_contract, e->weights_dev};

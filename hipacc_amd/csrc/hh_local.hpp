@@ -50,7 +50,8 @@ struct LocalArgs {
 //  tile    LDS-staged tile + halo, PPT rows per wave (hh_local_fast.hpp)
 //  stream  row-streaming strips, `param` rows per segment (hh_stream.hpp)
 //  generic one pixel per thread, any shape / mode (local.hip)
-enum Variant { kVarAuto = 0, kVarTile = 1, kVarStream = 2, kVarGeneric = 3 };
+//  wide    stream with 8 pixels per lane instead of 4
+enum Variant { kVarAuto = 0, kVarTile = 1, kVarStream = 2, kVarGeneric = 3, kVarStreamWide = 4 };
 
 // Operator kinds the tuner keys on.
 enum TuneKind { kTuneLocal = 0, kTuneSeparable = 1, kTuneBox = 2 };
@@ -87,8 +88,9 @@ struct SepArgs {
 };
 int launch_separable(const SepArgs& a, int acc_is_float, int fp_contract,
                      hipStream_t stream, int variant, int param);
-int dispatch_stream_local(const LocalArgs& a, int acc_is_float, int seg, hipStream_t s);
-int dispatch_stream_sep(const SepArgs& a, int acc_is_float, int fma, int seg, hipStream_t s);
+int dispatch_stream_local(const LocalArgs& a, int acc_is_float, int seg, int wide, hipStream_t s);
+int dispatch_stream_sep(const SepArgs& a, int acc_is_float, int fma, int seg, int wide,
+                        hipStream_t s);
 
 // ---- point operators ------------------------------------------------------
 struct PointArgs {

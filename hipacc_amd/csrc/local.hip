@@ -102,7 +102,9 @@ static int launch_local_variant(const LocalArgs& f, int acc_is_float, hipStream_
                                 int param) {
   switch (variant) {
     case kVarStream:
-      return dispatch_stream_local(f, acc_is_float, param, s);
+      return dispatch_stream_local(f, acc_is_float, param, 0, s);
+    case kVarStreamWide:
+      return dispatch_stream_local(f, acc_is_float, param, 1, s);
     case kVarTile: {
       if (!acc_is_float) return dispatch_sum_int(f, param, s);
       const int st = dispatch_sum_f32(f, param, s);

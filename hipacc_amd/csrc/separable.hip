@@ -211,7 +211,8 @@ static int launch_sep_generic(const SepArgs& a, int acc_is_float, int fp_contrac
 static int launch_sep_variant(const SepArgs& a, int acc_is_float, int fma, hipStream_t s,
                               int variant, int param) {
   switch (variant) {
-    case kVarStream: return dispatch_stream_sep(a, acc_is_float, fma, param, s);
+    case kVarStream: return dispatch_stream_sep(a, acc_is_float, fma, param, 0, s);
+    case kVarStreamWide: return dispatch_stream_sep(a, acc_is_float, fma, param, 1, s);
     case kVarTile: return dispatch_sep(a, acc_is_float, fma, param, s);
     case kVarGeneric: return launch_sep_generic(a, acc_is_float, fma, s);
   }

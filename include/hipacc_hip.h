@@ -266,6 +266,7 @@ int hipacc_hip_bilateral(const hipacc_bilateral_desc* desc, void* stream);
  *   2 stream  row-streaming strips with a register prefetch ring and
  *             cross-lane halos; param = output rows per wave segment
  *   3 generic one pixel per thread, any shape and reduction mode
+ *   4 wide    stream with 8 pixels per lane instead of 4
  * All variants compute identical results.  The plain entry points use the
  * tuned variant of the operator's shape when the tune cache has one, else
  * stream, else tile, else generic.  The tuner times every candidate on the
@@ -279,7 +280,7 @@ typedef struct hipacc_tune_entry {
   int32_t kind;        /* 0 local weighted SUM, 1 separable, 2 box sum         */
   int32_t kx, ky;
   int32_t in_dtype, out_dtype, acc_dtype, boundary, fp_contract, weights_dev;
-  int32_t variant;     /* 1 tile, 2 stream, 3 generic                         */
+  int32_t variant;     /* 1 tile, 2 stream, 3 generic, 4 wide stream          */
   int32_t param;
   float   ms;          /* measured time per launch                            */
 } hipacc_tune_entry;

@@ -13,8 +13,8 @@ from gpu_util import empty_gpu, to_gpu, to_host
 pytestmark = pytest.mark.gpu
 
 VARIANTS = [(capi.VAR_STREAM, 0), (capi.VAR_STREAM, 8), (capi.VAR_STREAM, 13),
-            (capi.VAR_STREAM, 64), (capi.VAR_TILE, 2), (capi.VAR_TILE, 4), (capi.VAR_TILE, 8),
-            (capi.VAR_GENERIC, 0)]
+            (capi.VAR_STREAM, 64), (capi.VAR_WIDE, 0), (capi.VAR_WIDE, 7), (capi.VAR_WIDE, 32),
+            (capi.VAR_TILE, 2), (capi.VAR_TILE, 4), (capi.VAR_TILE, 8), (capi.VAR_GENERIC, 0)]
 MODES = [capi.UNDEFINED, capi.CLAMP, capi.REPEAT, capi.MIRROR, capi.CONSTANT]
 
 
@@ -157,7 +157,7 @@ def test_tuner_records_a_winner_and_auto_uses_it(gpu, tmp_path):
     dst = empty_gpu(src.shape, np.float32)
     call = op.prepare_local(to_gpu(src), dst, 5, 5, weights=w, boundary=capi.CLAMP)
     e = call.tune(iters=3)
-    assert e.variant in (capi.VAR_TILE, capi.VAR_STREAM, capi.VAR_GENERIC) and e.ms > 0
+    assert e.variant in capi.VARIANT_NAMES and e.ms > 0
     path = str(tmp_path / "tune.txt").encode()
     capi.check(capi.lib().hipacc_hip_tune_save(path), "tune_save")
     capi.check(capi.lib().hipacc_hip_tune_load(path), "tune_load")
