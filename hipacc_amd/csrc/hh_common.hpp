@@ -89,18 +89,43 @@ HH_HD int bh_map(int mode, int i, int lo, int hi, int ext) {
 }
 
 // ---------------------------------------------------------------------------
-// Output conversions (hipacc_cvt).  Float -> integer: truncate toward zero to
-// int64 (NaN -> 0, saturated to the int64 range), then wrap to the width.
+// Output conversions (hipacc_cvt).  Float -> integer truncates toward zero
+// with saturation (NaN -> 0): to the uint32 range for uint outputs, to the
+// int32 range for every other integer output, which then wraps to its width.
+// This is the conversion instruction of the GPU (v_cvt_u32_f32 /
+// v_cvt_i32_f32); for every value the reference defines (in range of the
+// output type) it is the C cast.
 // ---------------------------------------------------------------------------
-HH_HD long long f2ll(float v) {
+HH_HD int32_t f2i_sat(float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  int32_t r;
+  asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+  return r;
+#else
   if (!(v == v)) return 0;
-  if (v >= 9.2233715e18f) return 0x7fffffffffffffffLL;
-  if (v <= -9.2233715e18f) return (long long)0x8000000000000000ULL;
-  return (long long)v;
+  if (v >= 2147483648.0f) return 2147483647;
+  if (v <= -2147483648.0f) return (int32_t)0x80000000u;
+  return (int32_t)v;
+#endif
+}
+HH_HD uint32_t f2u_sat(float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t r;
+  asm("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(v));
+  return r;
+#else
+  if (!(v == v) || v <= 0.0f) return 0u;
+  if (v >= 4294967296.0f) return 0xffffffffu;
+  return (uint32_t)v;
+#endif
 }
 template <typename TO> struct cvt_out {
-  HH_HD static TO from_f(float v) { return (TO)(uint64_t)f2ll(v); }
+  HH_HD static TO from_f(float v) { return (TO)f2i_sat(v); }
   HH_HD static TO from_i(int32_t v) { return (TO)v; }
+};
+template <> struct cvt_out<uint32_t> {
+  HH_HD static uint32_t from_f(float v) { return f2u_sat(v); }
+  HH_HD static uint32_t from_i(int32_t v) { return (uint32_t)v; }
 };
 template <> struct cvt_out<float> {
   HH_HD static float from_f(float v) { return v; }
@@ -152,6 +177,11 @@ template <bool FMA> HH_HD float madd(float w, float x, float acc) {
 template <bool FMA> HH_HD int32_t madd(int32_t w, int32_t x, int32_t acc) {
   return (int32_t)((uint32_t)acc + (uint32_t)w * (uint32_t)x);
 }
+
+// First tap of a sum: the product alone (the DSL seeds the result with the
+// first tap, dsl/kernel.hpp:250); integer products wrap.
+HH_HD float mul_first(float w, float x) { return w * x; }
+HH_HD int32_t mul_first(int32_t w, int32_t x) { return (int32_t)((uint32_t)w * (uint32_t)x); }
 
 // Reduction combine of hipacc::Reduce over the running value (dsl/kernel.hpp:
 // 253-260): MIN/MAX take (tap, acc) in that argument order.

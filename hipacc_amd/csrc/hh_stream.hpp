@@ -152,11 +152,14 @@ __device__ __forceinline__ void edge_row_load(TI (&v)[V], const Geom& g, const T
 // One streamed row: widen, gather the halo from the neighbour lanes, and add
 // the row into the accumulators of the KY output rows it belongs to.  `slot0`
 // is the accumulator of output row j (= j mod KY); `cst_row` (separable,
-// CONSTANT) makes the intermediate row the constant.
+// CONSTANT) makes the intermediate row the constant.  Only output rows j-dyi
+// with dyi <= maxdy exist (maxdy < KY-1 in the first rows of a segment; it
+// is a constant after unrolling, so the skipped taps cost nothing).
 template <typename TI, typename ACC, int V, int KX, int KY, bool FMA, int KIND, bool WDEV,
           typename ArgsT>
 __device__ __forceinline__ void stream_accumulate(const ArgsT& a, const TI (&raw)[V],
-                                                  ACC (&acc)[KY][V], int slot0, bool cst_row) {
+                                                  ACC (&acc)[KY][V], int slot0, bool cst_row,
+                                                  int maxdy) {
   constexpr int RX = KX / 2;
   constexpr int NWIN = V + 2 * RX;
   constexpr bool IS_F = std::is_same<ACC, float>::value;
@@ -174,16 +177,17 @@ __device__ __forceinline__ void stream_accumulate(const ArgsT& a, const TI (&raw
   if constexpr (KIND == kStream2D) {
 #pragma unroll
     for (int dyi = 0; dyi < KY; ++dyi) {
+      if (dyi > maxdy) break;
       const int s = (slot0 - dyi + KY) % KY;
-      if (dyi == 0) {
-#pragma unroll
-        for (int e = 0; e < V; ++e) acc[s][e] = ACC(0);
-      }
 #pragma unroll
       for (int dx = 0; dx < KX; ++dx) {
         const ACC w = StreamTaps<ArgsT>::template w2d<ACC, WDEV>(a, dyi * KX + dx);
 #pragma unroll
-        for (int e = 0; e < V; ++e) acc[s][e] = madd<FMA>(w, win[e + dx], acc[s][e]);
+        for (int e = 0; e < V; ++e) {
+          // the first tap of an output starts its sum: round(w*x) == fma(w, x, 0)
+          if (dyi == 0 && dx == 0) acc[s][e] = mul_first(w, win[e]);
+          else acc[s][e] = madd<FMA>(w, win[e + dx], acc[s][e]);
+        }
       }
     }
   } else {
@@ -203,7 +207,7 @@ __device__ __forceinline__ void stream_accumulate(const ArgsT& a, const TI (&raw
 #pragma unroll
         for (int dx = 0; dx < KX; ++dx) {
           const ACC w = IS_F ? (ACC)a.rowf[dx] : (ACC)a.rowi[dx];
-          s0 = madd<FMA>(w, win[e + dx], s0);
+          s0 = dx == 0 ? mul_first(w, win[e]) : madd<FMA>(w, win[e + dx], s0);
         }
         h[e] = s0;
       }
@@ -215,35 +219,39 @@ __device__ __forceinline__ void stream_accumulate(const ArgsT& a, const TI (&raw
     }
 #pragma unroll
     for (int dyi = 0; dyi < KY; ++dyi) {
+      if (dyi > maxdy) break;
       const int s = (slot0 - dyi + KY) % KY;
-      if (dyi == 0) {
-#pragma unroll
-        for (int e = 0; e < V; ++e) acc[s][e] = ACC(0);
-      }
       if constexpr (KIND == kStreamBox) {
 #pragma unroll
-        for (int e = 0; e < V; ++e) acc[s][e] = (ACC)((uint32_t)acc[s][e] + (uint32_t)h[e]);
+        for (int e = 0; e < V; ++e)
+          acc[s][e] = dyi == 0 ? h[e] : (ACC)((uint32_t)acc[s][e] + (uint32_t)h[e]);
       } else {
         const ACC w = IS_F ? (ACC)a.colf[dyi] : (ACC)a.coli[dyi];
 #pragma unroll
-        for (int e = 0; e < V; ++e) acc[s][e] = madd<FMA>(w, h[e], acc[s][e]);
+        for (int e = 0; e < V; ++e)
+          acc[s][e] = dyi == 0 ? mul_first(w, h[e]) : madd<FMA>(w, h[e], acc[s][e]);
       }
     }
   }
 }
 
-// Output row store of one lane (row in range; columns checked here).
+// Output row store of one lane (row in range; columns checked here).  The
+// streaming variants are only chosen without post_div and, for float
+// outputs, without rounding, so the conversion needs no division; full
+// vectors are written with nontemporal stores (the output is not re-read).
 template <typename TO, typename ACC, int V>
 __device__ __forceinline__ void stream_store(TO* __restrict__ orow, const Geom& g, bool full,
-                                             bool any, int xl, const ACC (&acc)[V]) {
+                                             bool any, int xl, float rnd, const ACC (&acc)[V]) {
   if (!any) return;
   TO vals[V];
 #pragma unroll
   for (int e = 0; e < V; ++e) {
-    if constexpr (std::is_same<ACC, float>::value)
-      vals[e] = convert_f<TO>((float)acc[e], g.cvt, g.post_div);
+    if constexpr (std::is_same<TO, float>::value)
+      vals[e] = (float)acc[e];
+    else if constexpr (std::is_same<ACC, float>::value)
+      vals[e] = cvt_out<TO>::from_f((float)acc[e] + rnd);
     else
-      vals[e] = convert_i<TO>((int32_t)acc[e], g.cvt, g.post_div);
+      vals[e] = cvt_out<TO>::from_i((int32_t)acc[e]);
   }
   TO* dst = orow + xl;
   if (full) {
@@ -255,19 +263,53 @@ __device__ __forceinline__ void stream_store(TO* __restrict__ orow, const Geom& 
   }
 }
 
-// Streams the wave's segment: prefetch PF rows, then blocks of PF rows (the
-// ring index u and the accumulator slot are compile-time constants inside a
-// block).  The first KY-1 rows complete no output.  Blocks whose refills all
-// stay inside the segment load without any clamp; in the last blocks the
-// refills past the last row re-read the last row, so every load is
-// unconditional and in bounds.
+// One block of PF streamed rows starting at row jb (the ring index u and the
+// accumulator slot are compile-time constants inside a block).  WARM: the
+// first block of the segment, whose row u only reaches outputs 0..u.
+template <typename TI, typename TO, typename ACC, int V, int KX, int KY, int PF, bool FMA,
+          int KIND, bool WDEV, bool FAST, bool WARM, typename ArgsT>
+__device__ __forceinline__ void stream_block(const ArgsT& a, const StreamCtx<TI, TO, V>& c,
+                                             TI (&ring)[PF][V], ACC (&acc)[KY][V], const TI* base,
+                                             TO* orow, bool full, bool any, float rnd, int jb) {
+  const Geom& g = a.g;
+  const bool cmode_sep = KIND != kStream2D && !FAST && g.boundary == HIPACC_BOUNDARY_CONSTANT;
+  const bool full_blk = jb + 2 * PF <= c.nj;  // every refill of this block is in range
+#pragma unroll
+  for (int u = 0; u < PF; ++u) {
+    const int j = jb + u;
+    if (!full_blk && j >= c.nj) break;
+    TI raw[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) raw[e] = ring[u][e];
+    const int r = j + PF;
+    if (full_blk || r < c.nj) {
+      if constexpr (FAST) load_px<TI, V>(ring[u], base + (long long)r * g.in_stride);
+      else edge_row_load<TI, V>(ring[u], g, c.in, c.cols, c.iy0 + r);
+    }
+    bool cst_row = false;
+    if (cmode_sep) {
+      const int iy = c.iy0 + j;
+      cst_row = iy < g.lo_y || iy >= g.hi_y;
+    }
+    stream_accumulate<TI, ACC, V, KX, KY, FMA, KIND, WDEV>(a, raw, acc, u % KY, cst_row,
+                                                           WARM ? u : KY - 1);
+    if (!WARM || u >= KY - 1)
+      stream_store<TO, ACC, V>(orow + (long long)j * g.out_stride, g, full, any, c.xl, rnd,
+                               acc[(u % KY + 1) % KY]);
+  }
+}
+
+// Streams the wave's segment: prefetch PF rows, then blocks of PF rows.  The
+// first KY-1 rows complete no output.  Blocks whose refills all stay inside
+// the segment load unconditionally; the last blocks skip the refills past the
+// segment's last row (a wave-uniform branch).
 template <typename TI, typename TO, typename ACC, int V, int KX, int KY, int PF, bool FMA,
           int KIND, bool WDEV, bool FAST, typename ArgsT>
 __device__ __forceinline__ void stream_rows(const ArgsT& a, const StreamCtx<TI, TO, V>& c) {
   const Geom& g = a.g;
-  const bool cmode_sep = KIND != kStream2D && !FAST && g.boundary == HIPACC_BOUNDARY_CONSTANT;
   const bool full = c.store_lane && a.out_vec_ok && c.xl + V <= g.is_w;
   const bool any = c.store_lane && c.xl < g.is_w;
+  const float rnd = g.cvt == HIPACC_CVT_ROUND ? 0.5f : 0.0f;
   TO* orow = c.out + (long long)(g.out_oy + c.y0 - (KY - 1)) * g.out_stride + g.out_ox;
   // FAST rows: wave-uniform row base plus this lane's fixed offset
   const TI* base = c.in + (long long)c.iy0 * g.in_stride + c.ix;
@@ -279,29 +321,11 @@ __device__ __forceinline__ void stream_rows(const ArgsT& a, const StreamCtx<TI, 
     if constexpr (FAST) load_px<TI, V>(ring[p], base + (long long)r * g.in_stride);
     else edge_row_load<TI, V>(ring[p], g, c.in, c.cols, c.iy0 + r);
   }
-  for (int jb = 0; jb < c.nj; jb += PF) {
-    const bool full_blk = jb + 2 * PF <= c.nj;  // every refill of this block is in range
-#pragma unroll
-    for (int u = 0; u < PF; ++u) {
-      const int j = jb + u;
-      if (!full_blk && j >= c.nj) break;
-      TI raw[V];
-#pragma unroll
-      for (int e = 0; e < V; ++e) raw[e] = ring[u][e];
-      const int r = full_blk ? j + PF : min(j + PF, c.nj - 1);
-      if constexpr (FAST) load_px<TI, V>(ring[u], base + (long long)r * g.in_stride);
-      else edge_row_load<TI, V>(ring[u], g, c.in, c.cols, c.iy0 + r);
-      bool cst_row = false;
-      if (cmode_sep) {
-        const int iy = c.iy0 + j;
-        cst_row = iy < g.lo_y || iy >= g.hi_y;
-      }
-      stream_accumulate<TI, ACC, V, KX, KY, FMA, KIND, WDEV>(a, raw, acc, u % KY, cst_row);
-      if (j >= KY - 1)
-        stream_store<TO, ACC, V>(orow + (long long)j * g.out_stride, g, full, any, c.xl,
-                                 acc[(u % KY + 1) % KY]);
-    }
-  }
+  stream_block<TI, TO, ACC, V, KX, KY, PF, FMA, KIND, WDEV, FAST, true>(a, c, ring, acc, base,
+                                                                        orow, full, any, rnd, 0);
+  for (int jb = PF; jb < c.nj; jb += PF)
+    stream_block<TI, TO, ACC, V, KX, KY, PF, FMA, KIND, WDEV, FAST, false>(a, c, ring, acc, base,
+                                                                           orow, full, any, rnd, jb);
 }
 
 template <typename TI, typename TO, typename ACC, int V, int KX, int KY, int PF, bool FMA,
@@ -367,7 +391,7 @@ static inline StreamGeom stream_geom(const Geom& g, int rx, int v, int seg_rows)
   const int hl = (rx + v - 1) / v;
   sg.strip_w = (64 - 2 * hl) * v;
   sg.strips = ceil_div(g.is_w, sg.strip_w);
-  sg.seg_rows = seg_rows > 0 ? seg_rows : 24;
+  sg.seg_rows = seg_rows > 0 ? seg_rows : 12;
   sg.segs = ceil_div(g.is_h, sg.seg_rows);
   sg.total_waves = sg.strips * sg.segs * g.batch;
   sg.xcd_perm = 0;
@@ -386,6 +410,9 @@ static inline dim3 stream_grid(StreamGeom& sg) {
 template <typename TI, typename TO, typename ACC, int V, int KX, int KY, int PF, bool FMA,
           int KIND, bool WDEV, typename ArgsT>
 static int launch_stream(const ArgsT& a0, int seg_rows, hipStream_t s) {
+  if (a0.g.post_div != 0.0f ||
+      (std::is_same<TO, float>::value && a0.g.cvt == HIPACC_CVT_ROUND))
+    return HIPACC_ERR_UNSUPPORTED;
   ArgsT a = a0;
   stream_align(a.g, V, a.in_vec_ok, a.out_vec_ok);
   StreamGeom sg = stream_geom(a.g, KX / 2, V, seg_rows);
@@ -395,7 +422,7 @@ static int launch_stream(const ArgsT& a0, int seg_rows, hipStream_t s) {
   return hip_status(hipGetLastError(), "stream_kernel launch");
 }
 
-// Prefetch depth in rows: at least ~8, a multiple of KY.
-#define HH_PF(K) ((K) >= 7 ? (K) : ((K) == 5 ? 10 : 9))
+// Prefetch depth in rows: a multiple of KY (at least 6).
+#define HH_PF(K) ((K) >= 5 ? (K) : 6)
 
 }  // namespace hh

@@ -25,22 +25,28 @@ template <> struct word_of<4> { using type = uint32_t; };
 template <> struct word_of<8> { using type = uint2; };
 template <> struct word_of<16> { using type = uint4; };
 
-// Stores VEC values of TO at dst, which is aligned to min(16, VEC*sizeof(TO)).
+// Stores VEC values of TO at dst, which is aligned to min(16, VEC*sizeof(TO)),
+// with nontemporal stores (operator outputs are not re-read by the kernel).
 template <typename TO, int VEC>
 __device__ __forceinline__ void store_vec(TO* dst, const TO (&v)[VEC]) {
   constexpr int BYTES = VEC * (int)sizeof(TO);
   if constexpr (BYTES >= 16) {
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 #pragma unroll
     for (int k = 0; k < BYTES / 16; ++k) {
-      uint4 w;
+      u32x4 w;
       __builtin_memcpy(&w, reinterpret_cast<const char*>(v) + 16 * k, 16);
-      reinterpret_cast<uint4*>(dst)[k] = w;
+      __builtin_nontemporal_store(w, reinterpret_cast<u32x4*>(dst) + k);
     }
   } else {
-    using W = typename word_of<BYTES>::type;
+    using W = typename std::conditional<
+        BYTES == 8, unsigned long long,
+        typename std::conditional<BYTES == 4, unsigned int,
+                                  typename std::conditional<BYTES == 2, unsigned short,
+                                                            unsigned char>::type>::type>::type;
     W w;
     __builtin_memcpy(&w, v, BYTES);
-    *reinterpret_cast<W*>(dst) = w;
+    __builtin_nontemporal_store(w, reinterpret_cast<W*>(dst));
   }
 }
 

@@ -75,9 +75,10 @@ typedef enum {
  * (dsl/types.hpp:115-117 convert<T>(float) == (T)s).  ROUND is the samples'
  * idiom `(uchar)(acc + 0.5f)` (e.g. samples-public/1_Local_Operators/
  * Gaussian_Blur/src/main.cpp:63-65).  Float-to-integer conversion truncates
- * toward zero to a 64-bit integer (NaN -> 0, saturating at the int64 range)
- * and then wraps to the output width, which is what x86-64 code does for every
- * value inside the int32 range; the reference leaves the rest undefined.    */
+ * toward zero; for values the output type can hold that is the C cast.  The
+ * reference leaves the rest undefined; here truncation saturates to the
+ * uint32 range for uint outputs and to the int32 range for every other
+ * integer output (NaN -> 0), which then wraps to the output width.          */
 typedef enum { HIPACC_CVT_TRUNC = 0, HIPACC_CVT_ROUND = 1 } hipacc_cvt;
 
 /* ---- an image window: an Accessor or an IterationSpace ----------------- */

@@ -68,26 +68,34 @@ static int32_t px_i(const void* p, int dt, long long i) {
   }
 }
 
-/* ---- output conversion: the C cast of dsl/types.hpp:115-117, with the
- * out-of-range cases pinned as in include/hipacc_hip.h (hipacc_cvt). ----- */
-static long long f2ll(float v) {
+/* ---- output conversion: the C cast of dsl/types.hpp:115-117 for every
+ * value the output type can hold; outside that range (undefined in the
+ * reference) truncation saturates to the uint32 range for uint outputs and
+ * to the int32 range otherwise (NaN -> 0), then wraps to the output width,
+ * as stated for hipacc_cvt in include/hipacc_hip.h. ----------------------- */
+static int32_t f2i_sat(float v) {
   if (!(v == v)) return 0;
-  if (v >= 9.2233715e18f) return 0x7fffffffffffffffLL;
-  if (v <= -9.2233715e18f) return (long long)0x8000000000000000ULL;
-  return (long long)v;
+  if (v >= 2147483648.0f) return 2147483647;
+  if (v <= -2147483648.0f) return (int32_t)0x80000000u;
+  return (int32_t)v;
+}
+static uint32_t f2u_sat(float v) {
+  if (!(v == v) || v <= 0.0f) return 0u;
+  if (v >= 4294967296.0f) return 0xffffffffu;
+  return (uint32_t)v;
 }
 static void px_put_f(void* p, int dt, long long i, float acc, int cvt, float post_div) {
   if (post_div != 0.0f) acc = acc / post_div;
   if (cvt == HIPACC_CVT_ROUND) acc = acc + 0.5f;
   if (dt == HIPACC_F32) { ((float*)p)[i] = acc; return; }
-  uint64_t u = (uint64_t)f2ll(acc);
+  const uint32_t u = (uint32_t)f2i_sat(acc);
   switch (dt) {
     case HIPACC_U8: ((uint8_t*)p)[i] = (uint8_t)u; break;
     case HIPACC_I8: ((int8_t*)p)[i] = (int8_t)(uint8_t)u; break;
     case HIPACC_U16: ((uint16_t*)p)[i] = (uint16_t)u; break;
     case HIPACC_I16: ((int16_t*)p)[i] = (int16_t)(uint16_t)u; break;
-    case HIPACC_U32: ((uint32_t*)p)[i] = (uint32_t)u; break;
-    case HIPACC_I32: ((int32_t*)p)[i] = (int32_t)(uint32_t)u; break;
+    case HIPACC_U32: ((uint32_t*)p)[i] = f2u_sat(acc); break;
+    case HIPACC_I32: ((int32_t*)p)[i] = (int32_t)u; break;
   }
 }
 static void px_put_i(void* p, int dt, long long i, int32_t acc, int cvt, float post_div) {
