@@ -11,7 +11,12 @@ separable 13x13 Gaussian, 7x7 box sum, reductions, histogram, bilateral) and
 reports them under ``ops``; the CPU baseline is the C oracle (OpenMP
 restatement of the reference's generated CPU backend) on a bounded sample.
 
+Before timing, the headline operator's kernel variant is chosen by timing
+each candidate (row-streaming segment heights, wide streaming, LDS tile) on
+the same rotating buffers; the choice is reported under roofline.variant.
+
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--quick]
+    python bench.py --profile [--variant stream:12]   # kernels for rocprofv3
 """
 from __future__ import annotations
 
@@ -56,6 +61,33 @@ def time_calls(calls, steps, warmup):
     return e0.elapsed_time(e1) / steps
 
 
+HEAD_CANDIDATES = ("stream:8", "stream:12", "stream:16", "stream:24", "wide:16", "tile:4")
+
+
+def parse_variant(text):
+    from hipacc_amd import capi
+    name, _, param = text.partition(":")
+    codes = {v: k for k, v in capi.VARIANT_NAMES.items()}
+    return codes[name], int(param or 0)
+
+
+def choose_variant(calls, steps=60):
+    """Times each candidate variant on the rotating buffers; returns
+    (ms, name, variant calls) of the fastest."""
+    from hipacc_amd import capi
+    best = None
+    for name in HEAD_CANDIDATES:
+        v, p = parse_variant(name)
+        vc = [c.with_variant(v, p) for c in calls]
+        try:
+            ms = min(time_calls(vc, steps, 5) for _ in range(2))
+        except capi.HipaccError:
+            continue
+        if best is None or ms < best[0]:
+            best = (ms, name, vc)
+    return best
+
+
 def op_entry(name, ms, bytes_per_px, npix=NPIX, extra=None):
     gbs = bytes_per_px * npix / (ms * 1e-3) / 1e9
     d = {"ms": round(ms, 5), "mpix_s": round(npix / (ms * 1e-3) / 1e6, 1),
@@ -96,6 +128,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--quick", action="store_true", help="headline operator only")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--profile", action="store_true",
+                    help="launch the headline kernel 20 times and a 64 MiB copy 20 times, "
+                         "for rocprofv3 counter runs; prints no result line")
+    ap.add_argument("--variant", default="stream:12",
+                    help="headline kernel variant under --profile (name:param)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -117,6 +154,19 @@ def main() -> int:
     g5 = masks.gaussian_2d(5, 1.0)
     head = [op.prepare_local(f32_in[i], f32_out[i], 5, 5, weights=g5, boundary=capi.CLAMP,
                              fp_contract=True) for i in range(NSETS)]
+    if args.profile:
+        v, p = parse_variant(args.variant)
+        calls = [c.with_variant(v, p) for c in head]
+        for i in range(20):
+            calls[i % NSETS].launch()
+        src = torch.empty(16 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(20):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"profile run: 20 x {args.variant} 5x5 f32 4096^2, 20 x 64 MiB copy", flush=True)
+        return 0
+    _, vname, head = choose_variant(head)
     samples = [time_calls(head, args.steps, args.warmup if k == 0 else 2) for k in range(5)]
     ms = statistics.median(samples)
     e = op_entry("conv5x5_f32_clamp", ms, 8)
@@ -139,7 +189,9 @@ def main() -> int:
                    "fp_contract": True},
         "roofline": {"bound": "hbm", "achieved": e["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": e["hbm_frac"], "traffic": None,
-                     "kernel": "local_sum_kernel<float,float,float,5,5,8,FMA>",
+                     "kernel": "stream_kernel" if vname.startswith(("stream", "wide"))
+                               else "local_sum_kernel",
+                     "variant": vname,
                      "bytes_per_launch": 8 * NPIX,
                      "batch_ms": [round(s, 5) for s in samples]},
     }
@@ -166,8 +218,18 @@ def main() -> int:
         g13 = masks.gaussian_1d(13, 3.0)
         ones7 = np.ones(7, np.int32)
 
-        def run(name, calls, bpp, n=steps, extra=None):
-            ops[name] = op_entry(name, time_calls(calls, n, 5), bpp, extra=extra)
+        def run(name, calls, bpp, n=steps, extra=None, npix=NPIX):
+            ops[name] = op_entry(name, time_calls(calls, n, 5), bpp, npix=npix, extra=extra)
+
+        # 1024x1024 uint8 5x5 Gaussian, rounded uint8 output (cache-resident:
+        # reported, no roofline claim)
+        s_in = [torch.empty((1024, 1024), dtype=torch.uint8, device=dev) for _ in range(NSETS)]
+        s_out = [torch.empty_like(t) for t in s_in]
+        for i, t in enumerate(s_in):
+            op.fill_random(t, 3000 + i)
+        run("gauss5x5_u8_1024_clamp_round",
+            [op.prepare_local(s_in[i], s_out[i], 5, 5, weights=g5, boundary=capi.CLAMP,
+                              cvt=capi.CVT_ROUND) for i in range(NSETS)], 2, npix=1024 * 1024)
 
         run("sobel3x3_f32_clamp", [op.prepare_local(f32_in[i], f32_out[i], 3, 3, weights=sob,
                                                     boundary=capi.CLAMP) for i in range(NSETS)], 8)

@@ -112,3 +112,14 @@ def test_init_without_device_reports_no_device():
     if torch.cuda.is_available():
         pytest.skip("a GPU is visible")
     assert capi.lib().hipacc_hip_init(0) == 4
+
+
+def test_cpp_runtime_compiles_and_links(tmp_path):
+    """The C++ runtime headers compile warning-free and a program using them
+    links against the library (it is run by the GPU tests)."""
+    exe = tmp_path / "runtime_sample"
+    p = subprocess.run(["g++", "-std=c++17", "-O0", "-Wall", "-Wextra", "-Werror",
+                        "-I", str(ROOT / "include"), str(ROOT / "tests" / "cpp" / "runtime_sample.cpp"),
+                        "-o", str(exe), "-L", str(ROOT / "hipacc_amd"), "-lhipacc_gfx950"],
+                       capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr
