@@ -63,6 +63,20 @@ class BilateralDesc(ctypes.Structure):
                 ("const_val", c_double), ("cvt", c_int32)]
 
 
+class LocalGradDesc(ctypes.Structure):
+    _fields_ = [("input", View), ("grad_out", View), ("grad_in", View), ("mask_w", c_int32),
+                ("mask_h", c_int32), ("weights_f32", c_void_p), ("weights_dev", c_void_p),
+                ("grad_weights_dev", c_void_p), ("boundary", c_int32), ("const_val", c_double),
+                ("workspace", c_void_p)]
+
+
+class BilateralGradDesc(ctypes.Structure):
+    _fields_ = [("input", View), ("output", View), ("norm", View), ("grad_out", View),
+                ("grad_in", View), ("ksize", c_int32), ("sigma_d", c_float), ("sigma_r", c_float),
+                ("spatial", c_void_p), ("boundary", c_int32), ("const_val", c_double),
+                ("grad_sigma_r_dev", c_void_p), ("workspace", c_void_p)]
+
+
 class TuneEntry(ctypes.Structure):
     _fields_ = [("kind", c_int32), ("kx", c_int32), ("ky", c_int32), ("in_dtype", c_int32),
                 ("out_dtype", c_int32), ("acc_dtype", c_int32), ("boundary", c_int32),
@@ -88,6 +102,10 @@ SIGNATURES = {
     "hipacc_hip_reduce_sync": (c_int32, [POINTER(View), c_int32, c_int32, c_void_p, c_void_p]),
     "hipacc_hip_histogram": (c_int32, [POINTER(View), c_int32, c_float, c_void_p, c_void_p]),
     "hipacc_hip_bilateral": (c_int32, [POINTER(BilateralDesc), c_void_p]),
+    "hipacc_hip_grad_workspace_bytes": (c_size_t, []),
+    "hipacc_hip_local_grad": (c_int32, [POINTER(LocalGradDesc), c_void_p]),
+    "hipacc_hip_bilateral_train": (c_int32, [POINTER(BilateralGradDesc), c_void_p]),
+    "hipacc_hip_bilateral_grad": (c_int32, [POINTER(BilateralGradDesc), c_void_p]),
     "hipacc_hip_alloc2d": (c_int32, [POINTER(c_void_p), POINTER(c_int32), c_int32, c_int32, c_int32,
                                      c_int32]),
     "hipacc_hip_free": (c_int32, [c_void_p]),

@@ -100,6 +100,19 @@ __global__ __launch_bounds__(256) void bilateral_kernel(const BilatArgs a) {
     for (int v = 0; v < VEC; ++v) r[v] = num[o][v] / den[o][v];
     store_row<TO, float, VEC>(out, g, a.out_vec_ok, ox, ty0 + wave * PPT + o, r);
   }
+  if constexpr (std::is_same<TO, float>::value) {
+    if (a.norm) {
+      float* nrm = a.norm + (long long)blockIdx.z * a.norm_plane;
+#pragma unroll
+      for (int o = 0; o < PPT; ++o) {
+        const int y = ty0 + wave * PPT + o;
+        if (y >= g.is_h) continue;
+#pragma unroll
+        for (int v = 0; v < VEC; ++v)
+          if (ox + v < g.is_w) nrm[(long long)y * a.norm_stride + ox + v] = den[o][v];
+      }
+    }
+  }
 }
 
 template <typename TI, typename TO, int KS>

@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "hh_grad.hpp"
 #include "hh_tile.hpp"
 
 namespace hh {
@@ -42,10 +43,13 @@ int hip_status(hipError_t e, const char* what) {
 constexpr int kMaxDevices = 64;
 constexpr size_t kRedWorkspace = 1024 * sizeof(unsigned long long) + 256;
 
+constexpr size_t kGradWorkspace = (size_t)kGradBlocks * kMaxTaps * sizeof(float);
+
 struct DeviceState {
   std::mutex mu;
   bool ready = false;
   void* red_ws = nullptr;
+  void* grad_ws = nullptr;   // adjoint partial sums, allocated on first use
 };
 static DeviceState g_dev[kMaxDevices];
 
@@ -68,6 +72,20 @@ static int ensure_workspace(int dev, void** ws) {
     if (e != hipSuccess) return hip_status(e, "hipDeviceSynchronize");
   }
   *ws = d.red_ws;
+  return HIPACC_OK;
+}
+
+static int ensure_grad_workspace(void** ws) {
+  int dev = 0;
+  int st = current_device(&dev);
+  if (st) return st;
+  DeviceState& d = g_dev[dev];
+  std::lock_guard<std::mutex> lk(d.mu);
+  if (!d.grad_ws) {
+    hipError_t e = hipMalloc(&d.grad_ws, kGradWorkspace);
+    if (e != hipSuccess) return hip_status(e, "hipMalloc(gradient workspace)");
+  }
+  *ws = d.grad_ws;
   return HIPACC_OK;
 }
 
@@ -137,6 +155,19 @@ static bool rows_aligned(const hipacc_view& v, int align) {
   if (((long long)v.offset_x * eb) % align) return false;
   if (batch_of(v) > 1 && (v.plane_stride * eb) % align) return false;
   return true;
+}
+
+// An F32 image viewed whole (the adjoint kernels' operands).
+static int grad_image(const hipacc_view& v, const char* name) {
+  int st = check_view(v, name);
+  if (st) return st;
+  if (v.dtype != HIPACC_F32) return set_error(HIPACC_ERR_UNSUPPORTED, "%s: F32 images only", name);
+  if (v.offset_x || v.offset_y || v.width != v.img_width || v.height != v.img_height)
+    return set_error(HIPACC_ERR_UNSUPPORTED, "%s: the window must be the whole image", name);
+  return HIPACC_OK;
+}
+static bool same_shape(const hipacc_view& a, const hipacc_view& b) {
+  return a.width == b.width && a.height == b.height && batch_of(a) == batch_of(b);
 }
 
 static void const_views(int dtype, double cv, float* cf, int* ci) {
@@ -686,6 +717,157 @@ int hipacc_hip_bilateral(const hipacc_bilateral_desc* d, void* stream) {
   hipStream_t s = static_cast<hipStream_t>(stream);
   Timer tm(s);
   return tm.finish(launch_bilateral(a, s));
+}
+
+size_t hipacc_hip_grad_workspace_bytes(void) { return kGradWorkspace; }
+
+int hipacc_hip_local_grad(const hipacc_local_grad_desc* d, void* stream) {
+  if (!d) return set_error(HIPACC_ERR_INVALID_ARG, "local_grad: desc is NULL");
+  const bool dx = d->grad_in.data != nullptr, dw = d->grad_weights_dev != nullptr;
+  int st = grad_image(d->in, "in");
+  if (st) return st;
+  if ((st = grad_image(d->grad_out, "grad_out"))) return st;
+  if (!same_shape(d->in, d->grad_out))
+    return set_error(HIPACC_ERR_INVALID_ARG, "local_grad: grad_out differs from in in size");
+  if (dx) {
+    if ((st = grad_image(d->grad_in, "grad_in"))) return st;
+    if (!same_shape(d->in, d->grad_in))
+      return set_error(HIPACC_ERR_INVALID_ARG, "local_grad: grad_in differs from in in size");
+  }
+  if (d->mask_w < 1 || d->mask_h < 1 || d->mask_w > HIPACC_MAX_MASK_DIM ||
+      d->mask_h > HIPACC_MAX_MASK_DIM || !(d->mask_w & 1) || !(d->mask_h & 1))
+    return set_error(HIPACC_ERR_INVALID_ARG, "local_grad: mask %dx%d (odd, <= %d)", d->mask_w,
+                     d->mask_h, HIPACC_MAX_MASK_DIM);
+  if (!d->weights_dev && !d->weights_f32)
+    return set_error(HIPACC_ERR_INVALID_ARG, "local_grad: no weights");
+  if (d->boundary < HIPACC_BOUNDARY_UNDEFINED || d->boundary > HIPACC_BOUNDARY_CONSTANT)
+    return set_error(HIPACC_ERR_INVALID_ARG, "local_grad: boundary %d", d->boundary);
+  ConvGradArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.x = static_cast<const float*>(d->in.data);
+  a.gy = static_cast<const float*>(d->grad_out.data);
+  a.gx = dx ? static_cast<float*>(d->grad_in.data) : nullptr;
+  a.x_plane = d->in.plane_stride;
+  a.gy_plane = d->grad_out.plane_stride;
+  a.gx_plane = dx ? d->grad_in.plane_stride : 0;
+  a.x_stride = d->in.stride;
+  a.gy_stride = d->grad_out.stride;
+  a.gx_stride = dx ? d->grad_in.stride : 0;
+  a.w = d->in.width;
+  a.h = d->in.height;
+  a.batch = batch_of(d->in);
+  a.kx = d->mask_w;
+  a.ky = d->mask_h;
+  a.boundary = d->boundary;
+  a.cval = (float)d->const_val;
+  a.vec_ok = rows_aligned(d->in, 16) && rows_aligned(d->grad_out, 16) &&
+             (!dx || rows_aligned(d->grad_in, 16));
+  a.wdev = d->weights_dev;
+  if (!a.wdev) std::memcpy(a.wh, d->weights_f32, sizeof(float) * a.kx * a.ky);
+  void* ws = d->workspace;
+  if (dw && !ws && (st = ensure_grad_workspace(&ws))) return st;
+  a.partials = static_cast<float*>(ws);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Timer tm(s);
+  return tm.finish(launch_conv_grad(a, d->grad_weights_dev, s));
+}
+
+// Spatial table and argument checks shared by the bilateral entry points.
+static int bilat_common(const hipacc_bilateral_grad_desc* d, float* sp) {
+  if (d->ksize < 1 || d->ksize > HIPACC_MAX_MASK_DIM || !(d->ksize & 1))
+    return set_error(HIPACC_ERR_INVALID_ARG, "bilateral: ksize %d", d->ksize);
+  if (!(d->sigma_r > 0.0f)) return set_error(HIPACC_ERR_INVALID_ARG, "bilateral: sigma_r must be > 0");
+  if (!d->spatial && !(d->sigma_d > 0.0f))
+    return set_error(HIPACC_ERR_INVALID_ARG, "bilateral: sigma_d must be > 0");
+  if (d->boundary < HIPACC_BOUNDARY_UNDEFINED || d->boundary > HIPACC_BOUNDARY_CONSTANT)
+    return set_error(HIPACC_ERR_INVALID_ARG, "bilateral: boundary %d", d->boundary);
+  const int r = d->ksize / 2;
+  for (int dy = 0; dy < d->ksize; ++dy)
+    for (int dx = 0; dx < d->ksize; ++dx) {
+      const int t = dy * d->ksize + dx;
+      if (d->spatial) {
+        sp[t] = d->spatial[t];
+      } else {
+        const double ddx = dx - r, ddy = dy - r;
+        sp[t] = (float)std::exp(-(ddx * ddx + ddy * ddy) / (2.0 * (double)d->sigma_d * d->sigma_d));
+      }
+    }
+  return HIPACC_OK;
+}
+
+int hipacc_hip_bilateral_train(const hipacc_bilateral_grad_desc* d, void* stream) {
+  if (!d) return set_error(HIPACC_ERR_INVALID_ARG, "bilateral_train: desc is NULL");
+  int st;
+  if ((st = grad_image(d->in, "in")) || (st = grad_image(d->out, "out")) ||
+      (st = grad_image(d->norm, "norm")))
+    return st;
+  if (!same_shape(d->in, d->out) || !same_shape(d->in, d->norm))
+    return set_error(HIPACC_ERR_INVALID_ARG, "bilateral_train: in, out and norm differ in size");
+  float sp[kMaxTaps];
+  if ((st = bilat_common(d, sp))) return st;
+  BilatArgs a;
+  std::memset(&a, 0, sizeof(a));
+  if ((st = make_geom(a.g, d->in, d->out, d->boundary, d->const_val, HIPACC_CVT_TRUNC, 0.0f)))
+    return st;
+  a.ksize = d->ksize;
+  for (int t = 0; t < d->ksize * d->ksize; ++t) a.lsp[t] = sp[t] > 0.0f ? std::log2(sp[t]) : -INFINITY;
+  const float c_r = 0.5f / (d->sigma_r * d->sigma_r);
+  a.k2 = -c_r * 1.44269504088896340736f;
+  a.in_vec_ok = rows_aligned(d->in, 16);
+  a.out_vec_ok = rows_aligned(d->out, 16);
+  a.norm = static_cast<float*>(d->norm.data);
+  a.norm_plane = d->norm.plane_stride;
+  a.norm_stride = d->norm.stride;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Timer tm(s);
+  return tm.finish(launch_bilateral(a, s));
+}
+
+int hipacc_hip_bilateral_grad(const hipacc_bilateral_grad_desc* d, void* stream) {
+  if (!d) return set_error(HIPACC_ERR_INVALID_ARG, "bilateral_grad: desc is NULL");
+  const bool dx = d->grad_in.data != nullptr, ds = d->grad_sigma_r_dev != nullptr;
+  int st;
+  if ((st = grad_image(d->in, "in")) || (st = grad_image(d->out, "out")) ||
+      (st = grad_image(d->norm, "norm")) || (st = grad_image(d->grad_out, "grad_out")))
+    return st;
+  if (dx && (st = grad_image(d->grad_in, "grad_in"))) return st;
+  if (!same_shape(d->in, d->out) || !same_shape(d->in, d->norm) ||
+      !same_shape(d->in, d->grad_out) || (dx && !same_shape(d->in, d->grad_in)))
+    return set_error(HIPACC_ERR_INVALID_ARG, "bilateral_grad: images differ in size");
+  BilatGradArgs a;
+  std::memset(&a, 0, sizeof(a));
+  if ((st = bilat_common(d, a.sp))) return st;
+  a.x = static_cast<const float*>(d->in.data);
+  a.y = static_cast<const float*>(d->out.data);
+  a.d = static_cast<const float*>(d->norm.data);
+  a.gy = static_cast<const float*>(d->grad_out.data);
+  a.gx = dx ? static_cast<float*>(d->grad_in.data) : nullptr;
+  a.x_plane = d->in.plane_stride;
+  a.y_plane = d->out.plane_stride;
+  a.d_plane = d->norm.plane_stride;
+  a.gy_plane = d->grad_out.plane_stride;
+  a.gx_plane = dx ? d->grad_in.plane_stride : 0;
+  a.x_stride = d->in.stride;
+  a.y_stride = d->out.stride;
+  a.d_stride = d->norm.stride;
+  a.gy_stride = d->grad_out.stride;
+  a.gx_stride = dx ? d->grad_in.stride : 0;
+  a.w = d->in.width;
+  a.h = d->in.height;
+  a.batch = batch_of(d->in);
+  a.ksize = d->ksize;
+  a.boundary = d->boundary;
+  a.cval = (float)d->const_val;
+  a.c = 0.5f / (d->sigma_r * d->sigma_r);
+  a.k2 = -a.c * 1.44269504088896340736f;
+  a.vec_ok = rows_aligned(d->in, 16) && rows_aligned(d->out, 16) && rows_aligned(d->norm, 16) &&
+             rows_aligned(d->grad_out, 16) && (!dx || rows_aligned(d->grad_in, 16));
+  void* ws = d->workspace;
+  if (!ws && (st = ensure_grad_workspace(&ws))) return st;
+  a.partials = static_cast<float*>(ws);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Timer tm(s);
+  return tm.finish(launch_bilateral_grad(a, ds ? d->grad_sigma_r_dev : nullptr, d->sigma_r, s));
 }
 
 int hipacc_hip_alloc2d(void** ptr, int32_t* stride_elems, int32_t width, int32_t height,

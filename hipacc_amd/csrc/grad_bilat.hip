@@ -1,0 +1,250 @@
+// grad_bilat.hip -- backward of the bilateral filter with a learnable range
+// sigma.  Forward (per output pixel p, taps t at offsets o_t, v = X_bh(p+o_t)):
+//     s = exp(-c (v - X(p))^2) * sp[t],  D(p) = SUM s,  Y(p) = SUM s*v / D(p)
+// With a(p) = dY(p)/D(p), d = v - X(q):
+//   centre term of q      2c a(q) SUM_t s_{q,t} d (v - Y(q))
+//   neighbour terms of q  SUM a(p) s_{p,t} [1 + 2c d' (X(q) - Y(p))] over the
+//                         (p, t) whose tap reads q, d' = X(p) - X(q)
+//   dL/dc                 -SUM_q a(q) SUM_t s_{q,t} d^2 (v - Y(q))
+// Away from the borders the neighbour p = q + o reads q through the opposite
+// tap -o, and exp(-c d^2) is the same for the pair, so one exponential per
+// (q, o) serves both terms.  Pixels within r of an edge add the neighbour
+// terms of the halo positions the boundary mode folds onto them.
+#include "hh_grad.hpp"
+
+namespace hh {
+
+// Folded neighbour terms of border pixel q: taps of in-image pixels p that
+// land on an extended position i != q with bh(i) == q.  Global reads.
+__device__ float bilat_fold(const BilatGradArgs& a, int b, int qx, int qy, float xq) {
+  int px[2 * (HIPACC_MAX_MASK_DIM / 2) + 1], py[2 * (HIPACC_MAX_MASK_DIM / 2) + 1];
+  const int r = a.ksize / 2;
+  const int nx = preimage(a.boundary, qx, a.w, r, px);
+  const int ny = preimage(a.boundary, qy, a.h, r, py);
+  const float* X = a.x + b * a.x_plane;
+  const float* Y = a.y + b * a.y_plane;
+  const float* D = a.d + b * a.d_plane;
+  const float* G = a.gy + b * a.gy_plane;
+  float acc = 0.0f;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      if (i == 0 && j == 0) continue;
+      for (int ty = 0; ty < a.ksize; ++ty) {
+        const int y = py[j] - (ty - r);
+        if (y < 0 || y >= a.h) continue;
+        for (int tx = 0; tx < a.ksize; ++tx) {
+          const int x = px[i] - (tx - r);
+          if (x < 0 || x >= a.w) continue;
+          const float xp = X[(long long)y * a.x_stride + x];
+          const float yp = Y[(long long)y * a.y_stride + x];
+          const float ap = G[(long long)y * a.gy_stride + x] / D[(long long)y * a.d_stride + x];
+          const float dd = xp - xq;
+          const float s = __builtin_amdgcn_exp2f(a.k2 * (dd * dd)) * a.sp[ty * a.ksize + tx];
+          const float w = ap * s;
+          acc += w + 2.0f * a.c * (w * dd) * (xq - yp);
+        }
+      }
+    }
+  return acc;
+}
+
+// 64 x 16 pixel tiles, 4 adjacent pixels per lane.  LDS holds, with a halo of
+// r rows and RH columns: X_bh (CONSTANT: the constant), a = dY/D (0 outside
+// the image) and Y.
+template <int R, bool DX>
+__global__ __launch_bounds__(256) void bilat_grad_kernel(const BilatGradArgs a) {
+  constexpr int K = 2 * R + 1, KK = K * K;
+  constexpr int TW = 64, TH = 16;
+  constexpr int RH = (R + 3) & ~3;
+  constexpr int LW = TW + 2 * RH, LH = TH + 2 * R;
+  constexpr int M = 4 + 2 * R;
+  __shared__ __align__(16) float xs[LH * LW];
+  __shared__ __align__(16) float as[LH * LW];
+  __shared__ __align__(16) float ys[LH * LW];
+  __shared__ float red[4];
+
+  const int tid = threadIdx.x, tr = tid >> 4, tc = (tid & 15) * 4;
+  const int per_plane = a.tiles_x * a.tiles_y;
+  const float k2 = a.k2, c2 = 2.0f * a.c;
+  float dc = 0.0f;
+
+  for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    const int b = tile / per_plane, rem = tile - b * per_plane;
+    const int y0 = (rem / a.tiles_x) * TH, x0 = (rem % a.tiles_x) * TW;
+    const float* X = a.x + b * a.x_plane;
+    const float* Y = a.y + b * a.y_plane;
+    const float* D = a.d + b * a.d_plane;
+    const float* G = a.gy + b * a.gy_plane;
+    const bool interior =
+        a.vec_ok && x0 >= RH && x0 + TW + RH <= a.w && y0 >= R && y0 + TH + R <= a.h;
+    __syncthreads();
+    if (interior) {
+      constexpr int LW4 = LW / 4, N4 = LH * LW4;
+      for (int i = tid; i < N4; i += 256) {
+        const int r = i / LW4, c = (i - r * LW4) * 4;
+        const long long gx = x0 - RH + c;
+        const int gy = y0 - R + r;
+        const float4 xv = *reinterpret_cast<const float4*>(X + (long long)gy * a.x_stride + gx);
+        const float4 yv = *reinterpret_cast<const float4*>(Y + (long long)gy * a.y_stride + gx);
+        const float4 dv = *reinterpret_cast<const float4*>(D + (long long)gy * a.d_stride + gx);
+        const float4 gv = *reinterpret_cast<const float4*>(G + (long long)gy * a.gy_stride + gx);
+        *reinterpret_cast<float4*>(xs + r * LW + c) = xv;
+        *reinterpret_cast<float4*>(ys + r * LW + c) = yv;
+        *reinterpret_cast<float4*>(as + r * LW + c) =
+            make_float4(gv.x / dv.x, gv.y / dv.y, gv.z / dv.z, gv.w / dv.w);
+      }
+    } else {
+      for (int i = tid; i < LH * LW; i += 256) {
+        const int r = i / LW, c = i - r * LW;
+        const int y = y0 - R + r, x = x0 - RH + c;
+        const bool inside = x >= 0 && x < a.w && y >= 0 && y < a.h;
+        float xv, av = 0.0f, yv = 0.0f;
+        if (inside) {
+          xv = X[(long long)y * a.x_stride + x];
+          yv = Y[(long long)y * a.y_stride + x];
+          av = G[(long long)y * a.gy_stride + x] / D[(long long)y * a.d_stride + x];
+        } else if (a.boundary == HIPACC_BOUNDARY_CONSTANT) {
+          xv = a.cval;
+        } else {
+          xv = X[(long long)bh_map(a.boundary, y, 0, a.h, a.h) * a.x_stride +
+                 bh_map(a.boundary, x, 0, a.w, a.w)];
+        }
+        xs[i] = xv;
+        as[i] = av;
+        ys[i] = yv;
+      }
+    }
+    __syncthreads();
+
+    float xq[4], yq[4], aq[4];
+    float accc[4] = {0.0f, 0.0f, 0.0f, 0.0f}, accs[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float accn[4] = {0.0f, 0.0f, 0.0f, 0.0f}, accm[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int o = (tr + R) * LW + tc + RH + j;
+      xq[j] = xs[o];
+      yq[j] = ys[o];
+      aq[j] = as[o];
+    }
+    // One tap row per iteration (not unrolled): the three row windows stay
+    // within the register budget of 4 waves per SIMD.
+#pragma unroll 1
+    for (int dy = 0; dy < K; ++dy) {
+      float xr[M], ar[DX ? M : 1], yr[DX ? M : 1];
+      const int off = (tr + dy) * LW + tc + (RH - R);
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        xr[m] = xs[off + m];
+        if constexpr (DX) {
+          ar[m] = as[off + m];
+          yr[m] = ys[off + m];
+        }
+      }
+#pragma unroll
+      for (int dx = 0; dx < K; ++dx) {
+        const float spc = a.sp[dy * K + dx];           // tap o of q
+        const float spn = a.sp[KK - 1 - (dy * K + dx)]; // tap -o of p = q + o
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float v = xr[j + dx];
+          const float d = v - xq[j];
+          const float e = __builtin_amdgcn_exp2f(k2 * (d * d));
+          const float u = (e * spc) * d * (v - yq[j]);
+          accc[j] += u;
+          accs[j] = __builtin_fmaf(u, d, accs[j]);
+          if constexpr (DX) {
+            const float w = ar[j + dx] * (e * spn);
+            accn[j] += w;
+            accm[j] = __builtin_fmaf(w * d, xq[j] - yr[j + dx], accm[j]);
+          }
+        }
+      }
+    }
+
+    const int qy = y0 + tr;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int qx = x0 + tc + j;
+      if (qy < a.h && qx < a.w) dc -= aq[j] * accs[j];
+    }
+    if constexpr (DX) {
+      if (qy < a.h) {
+        float* GX = a.gx + b * a.gx_plane + (long long)qy * a.gx_stride;
+        float v[4];
+        // neighbour term: SUM w [1 + 2c (X(p) - X(q)) (X(q) - Y(p))], d = X(p) - X(q)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = c2 * aq[j] * accc[j] + (accn[j] + c2 * accm[j]);
+        if (interior) {
+          *reinterpret_cast<float4*>(GX + x0 + tc) = make_float4(v[0], v[1], v[2], v[3]);
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int qx = x0 + tc + j;
+            if (qx >= a.w) continue;
+            float r = v[j];
+            if (a.boundary != HIPACC_BOUNDARY_CONSTANT &&
+                (qx < R || qx >= a.w - R || qy < R || qy >= a.h - R))
+              r += bilat_fold(a, b, qx, qy, xq[j]);
+            GX[qx] = r;
+          }
+        }
+      }
+    }
+  }
+
+  dc = wave_sum(dc);
+  if ((tid & 63) == 0) red[tid >> 6] = dc;
+  __syncthreads();
+  if (tid == 0) a.partials[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+// dsigma = -1/sigma^3 * SUM_b partials[b], fixed order.
+__global__ __launch_bounds__(256) void bilat_grad_reduce(const float* partials, int nblocks,
+                                                         float scale, float* dsigma) {
+  __shared__ float red[4];
+  float s = 0.0f;
+  for (int b = threadIdx.x; b < nblocks; b += 256) s += partials[b];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) *dsigma = (((red[0] + red[1]) + red[2]) + red[3]) * scale;
+}
+
+template <int R>
+static int launch_bg(const BilatGradArgs& a, int nblocks, hipStream_t s) {
+  if (a.gx)
+    hipLaunchKernelGGL((bilat_grad_kernel<R, true>), dim3(nblocks), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((bilat_grad_kernel<R, false>), dim3(nblocks), dim3(256), 0, s, a);
+  return hip_status(hipGetLastError(), "bilat_grad_kernel launch");
+}
+
+int launch_bilateral_grad(const BilatGradArgs& a0, float* dsigma, float sigma_r, hipStream_t s) {
+  BilatGradArgs a = a0;
+  if (!a.gx && !dsigma) return HIPACC_OK;
+  a.tiles_x = ceil_div(a.w, 64);
+  a.tiles_y = ceil_div(a.h, 16);
+  a.ntiles = a.tiles_x * a.tiles_y * a.batch;
+  const int nblocks = a.ntiles < kGradBlocks ? a.ntiles : kGradBlocks;
+  int st;
+  switch (a.ksize) {
+    case 1: st = launch_bg<0>(a, nblocks, s); break;
+    case 3: st = launch_bg<1>(a, nblocks, s); break;
+    case 5: st = launch_bg<2>(a, nblocks, s); break;
+    case 7: st = launch_bg<3>(a, nblocks, s); break;
+    case 9: st = launch_bg<4>(a, nblocks, s); break;
+    case 11: st = launch_bg<5>(a, nblocks, s); break;
+    case 13: st = launch_bg<6>(a, nblocks, s); break;
+    case 15: st = launch_bg<7>(a, nblocks, s); break;
+    default: return set_error(HIPACC_ERR_UNSUPPORTED, "bilateral_grad: ksize %d", a.ksize);
+  }
+  if (st == HIPACC_OK && dsigma) {
+    const double sr = sigma_r;
+    hipLaunchKernelGGL(bilat_grad_reduce, dim3(1), dim3(256), 0, s, a.partials, nblocks,
+                       (float)(-1.0 / (sr * sr * sr)), dsigma);
+    st = hip_status(hipGetLastError(), "bilat_grad_reduce launch");
+  }
+  return st;
+}
+
+}  // namespace hh

@@ -143,6 +143,9 @@ struct BilatArgs {
   int in_vec_ok, out_vec_ok;
   float k2;                 // -c_r * log2(e)
   float lsp[kMaxTaps];      // log2(spatial)
+  float* norm;              // F32 out only: per-pixel SUM of weights, or NULL
+  long long norm_plane;
+  int norm_stride;
 };
 int launch_bilateral(const BilatArgs& a, hipStream_t s);
 

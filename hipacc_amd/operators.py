@@ -252,6 +252,81 @@ def prepare_bilateral(src: torch.Tensor, dst: torch.Tensor, ksize: int, sigma_d:
     return Call("hipacc_hip_bilateral", d, [keep])
 
 
+def prepare_local_grad(src: torch.Tensor, grad_out: torch.Tensor,
+                       grad_in: Optional[torch.Tensor], mask_w: int, mask_h: int, *,
+                       weights: Optional[Sequence[float]] = None,
+                       weights_dev: Optional[torch.Tensor] = None,
+                       grad_weights: Optional[torch.Tensor] = None, boundary: int = capi.CLAMP,
+                       const_val: float = 0.0,
+                       workspace: Optional[torch.Tensor] = None) -> Call:
+    """Adjoint of the float weighted-SUM local operator: dL/dX into grad_in
+    and dL/dw into grad_weights (a device float tensor of mask_h*mask_w)."""
+    d = capi.LocalGradDesc()
+    d.input = view(src)
+    d.grad_out = view(grad_out)
+    if grad_in is not None:
+        d.grad_in = view(grad_in)
+    d.mask_w, d.mask_h = mask_w, mask_h
+    keep = []
+    if weights_dev is not None:
+        d.weights_dev = weights_dev.data_ptr()
+        keep.append(weights_dev)
+    elif weights is not None:
+        a, d.weights_f32 = _arr(weights, np.float32)
+        keep.append(a)
+    if grad_weights is not None:
+        d.grad_weights_dev = grad_weights.data_ptr()
+    d.boundary, d.const_val = boundary, const_val
+    if workspace is not None:
+        d.workspace = workspace.data_ptr()
+    return Call("hipacc_hip_local_grad", d, keep)
+
+
+def _bilateral_grad_desc(src, dst, norm, ksize, sigma_d, sigma_r, spatial, boundary, const_val):
+    d = capi.BilateralGradDesc()
+    d.input = view(src)
+    d.output = view(dst)
+    d.norm = view(norm)
+    d.ksize, d.sigma_d, d.sigma_r = ksize, sigma_d, sigma_r
+    keep = []
+    if spatial is not None:
+        a, d.spatial = _arr(spatial, np.float32)
+        keep.append(a)
+    d.boundary, d.const_val = boundary, const_val
+    return d, keep
+
+
+def prepare_bilateral_train(src: torch.Tensor, dst: torch.Tensor, norm: torch.Tensor, ksize: int,
+                            sigma_d: float, sigma_r: float, *,
+                            spatial: Optional[Sequence[float]] = None,
+                            boundary: int = capi.MIRROR, const_val: float = 0.0) -> Call:
+    """Bilateral forward that also writes the per-pixel normaliser."""
+    d, keep = _bilateral_grad_desc(src, dst, norm, ksize, sigma_d, sigma_r, spatial, boundary,
+                                   const_val)
+    return Call("hipacc_hip_bilateral_train", d, keep)
+
+
+def prepare_bilateral_grad(src: torch.Tensor, dst: torch.Tensor, norm: torch.Tensor,
+                           grad_out: torch.Tensor, grad_in: Optional[torch.Tensor], ksize: int,
+                           sigma_d: float, sigma_r: float, *,
+                           grad_sigma: Optional[torch.Tensor] = None,
+                           spatial: Optional[Sequence[float]] = None,
+                           boundary: int = capi.MIRROR, const_val: float = 0.0,
+                           workspace: Optional[torch.Tensor] = None) -> Call:
+    """Bilateral backward: dL/dX into grad_in, dL/dsigma_r into grad_sigma
+    (a one-element device float tensor)."""
+    d, keep = _bilateral_grad_desc(src, dst, norm, ksize, sigma_d, sigma_r, spatial, boundary,
+                                   const_val)
+    d.grad_out = view(grad_out)
+    if grad_in is not None:
+        d.grad_in = view(grad_in)
+    if grad_sigma is not None:
+        d.grad_sigma_r_dev = grad_sigma.data_ptr()
+    if workspace is not None:
+        d.workspace = workspace.data_ptr()
+    return Call("hipacc_hip_bilateral_grad", d, keep)
+
+
 def fill_random(t: torch.Tensor, seed: int, window: Window = None) -> None:
     v = view(t, window)
     capi.check(capi.lib().hipacc_hip_fill_random(ctypes.byref(v), seed, _stream()),

@@ -261,6 +261,69 @@ typedef struct hipacc_bilateral_desc {
 } hipacc_bilateral_desc;
 int hipacc_hip_bilateral(const hipacc_bilateral_desc* desc, void* stream);
 
+/* ---- adjoint of a learnable float convolution ------------------------- */
+/* The forward is the float weighted SUM of hipacc_hip_local over whole
+ * images (accessor window = image, iteration space of the same size):
+ *     Y(p) = SUM_t w[t] * X_bh(p + t - r)          (dsl/kernel.hpp:241-267)
+ * Given dY = dL/dY this computes
+ *     dX(q) = SUM_{p,t : bh(p + t - r) = q} w[t] * dY(p)
+ *     dw[t] = SUM_p dY(p) * X_bh(p + t - r)        (summed over the batch)
+ * in gather form: a flipped-mask stencil over dY whose border pixels add the
+ * halo gradients their boundary mode folds onto them (CLAMP, MIRROR, REPEAT;
+ * CONSTANT halos take no gradient but const_val enters dw; UNDEFINED reads
+ * are clamped to the image, so it folds as CLAMP).  dw is a fixed-order
+ * reduction (deterministic).  F32 images; in, grad_out and grad_in have the
+ * same size and batch.  No counterpart in the reference (it has no adjoints).
+ * workspace: NULL (the library's per-device buffer) or device memory of
+ * hipacc_hip_grad_workspace_bytes() bytes.                                  */
+typedef struct hipacc_local_grad_desc {
+  hipacc_view in;            /* X, the forward input                        */
+  hipacc_view grad_out;      /* dL/dY                                       */
+  hipacc_view grad_in;       /* dL/dX, overwritten; data NULL = skip        */
+  int32_t mask_w, mask_h;    /* odd, 1..HIPACC_MAX_MASK_DIM                 */
+  const float* weights_f32;  /* host mask, or                               */
+  const float* weights_dev;  /* device mask (takes precedence)              */
+  float*  grad_weights_dev;  /* device, mask_h*mask_w floats, overwritten;
+                                NULL = skip                                 */
+  int32_t boundary;
+  double  const_val;
+  void*   workspace;
+} hipacc_local_grad_desc;
+size_t hipacc_hip_grad_workspace_bytes(void);
+int hipacc_hip_local_grad(const hipacc_local_grad_desc* desc, void* stream);
+
+/* ---- bilateral filter with a learnable range sigma -------------------- */
+/* Training forward: hipacc_hip_bilateral (F32, whole images) that also
+ * writes the normaliser D(p) = SUM_t s_t, s_t = exp(-c*(v_t - X(p))^2) *
+ * spatial[t], c = 0.5/sigma_r^2, so Y(p) = SUM_t s_t*v_t / D(p).
+ * Backward: given dY, with a(p) = dY(p)/D(p),
+ *   dX(q) = 2c*a(q) * SUM_t s_{q,t} (v_t - X(q)) (v_t - Y(q))         (centre)
+ *         + SUM_{p,t: bh(p+t-r) = q} a(p) s_{p,t} [1 - 2c (X(q) - X(p)) (X(q) - Y(p))]
+ *   dL/dsigma_r = (-1/sigma_r^3) * dL/dc,
+ *   dL/dc = -SUM_p a(p) SUM_t s_{p,t} (v_t - X(p))^2 (v_t - Y(p)),
+ * recomputing the range weights (nothing per tap is stored); the second dX
+ * sum is gathered per q with the boundary folding of hipacc_hip_local_grad;
+ * the range factor exp(-c*d^2) of a pixel pair is computed once and serves
+ * both the centre term of q and the neighbour term of p.  No counterpart in
+ * the reference.                                                            */
+typedef struct hipacc_bilateral_grad_desc {
+  hipacc_view in;            /* X                                           */
+  hipacc_view out;           /* Y (written by train, read by grad)          */
+  hipacc_view norm;          /* D (written by train, read by grad)          */
+  hipacc_view grad_out;      /* dL/dY (grad)                                */
+  hipacc_view grad_in;       /* dL/dX (grad, overwritten); data NULL = skip */
+  int32_t ksize;             /* odd, 1..HIPACC_MAX_MASK_DIM                 */
+  float   sigma_d;
+  float   sigma_r;
+  const float* spatial;      /* host table or NULL (from sigma_d)           */
+  int32_t boundary;
+  double  const_val;
+  float*  grad_sigma_r_dev;  /* device, 1 float, overwritten; NULL = skip   */
+  void*   workspace;         /* NULL or hipacc_hip_grad_workspace_bytes()   */
+} hipacc_bilateral_grad_desc;
+int hipacc_hip_bilateral_train(const hipacc_bilateral_grad_desc* desc, void* stream);
+int hipacc_hip_bilateral_grad(const hipacc_bilateral_grad_desc* desc, void* stream);
+
 /* ---- kernel variants and the autotuner ------------------------------- */
 /* A local or separable operator runs one of three kernel variants:
  *   1 tile    LDS-staged tile + halo; param = output rows per wave (2/4/8)

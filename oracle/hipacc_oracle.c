@@ -625,6 +625,135 @@ ORC_EXPORT int oracle_bilateral_f64(const hipacc_bilateral_desc* d, double* out_
   return 0;
 }
 
+/* ---- adjoint of the float weighted SUM (no reference counterpart: the
+ * reference has no gradients).  Scatter form, the literal transpose of the
+ * forward's index map (dsl/kernel.hpp:241-267 with the boundary maps of
+ * lib/AST/BorderHandling.cpp:41-120): every tap of every output pixel sends
+ * w[t]*dY(p) to the input pixel it read and dY(p)*X to dw[t].  In double;
+ * dx is dense [batch][h][w].  dx_abs / dw_abs (optional) receive the sums of
+ * the absolute values of the terms, the scale of each result's rounding. */
+ORC_EXPORT int oracle_local_grad(const hipacc_local_grad_desc* d, double* dx, double* dw,
+                                 double* dx_abs, double* dw_abs) {
+  const hipacc_view* in = &d->in;
+  const hipacc_view* gy = &d->grad_out;
+  const int W = in->width, H = in->height, kx = d->mask_w, ky = d->mask_h;
+  const int rx = kx / 2, ry = ky / 2, nb = nbatch(in);
+  const float* w = d->weights_f32;
+  const double cval = (double)(float)d->const_val;
+  memset(dx, 0, sizeof(double) * (size_t)nb * W * H);
+  memset(dw, 0, sizeof(double) * kx * ky);
+  if (dx_abs) memset(dx_abs, 0, sizeof(double) * (size_t)nb * W * H);
+  if (dw_abs) memset(dw_abs, 0, sizeof(double) * kx * ky);
+  for (int b = 0; b < nb; ++b) {
+    const float* X = (const float*)plane(in, b);
+    const float* G = (const float*)plane(gy, b);
+    double* DX = dx + (size_t)b * W * H;
+    double* DXA = dx_abs ? dx_abs + (size_t)b * W * H : NULL;
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < W; ++x) {
+        const double g = G[(long long)y * gy->stride + x];
+        for (int ty = 0; ty < ky; ++ty)
+          for (int tx = 0; tx < kx; ++tx) {
+            const int t = ty * kx + tx;
+            const int sx = x + tx - rx, sy = y + ty - ry;
+            if (d->boundary == HIPACC_BOUNDARY_CONSTANT &&
+                (sx < 0 || sx >= W || sy < 0 || sy >= H)) {
+              dw[t] += g * cval;
+              if (dw_abs) dw_abs[t] += fabs(g * cval);
+              continue;
+            }
+            const int ix = bh_index(d->boundary, sx, 0, W, W);
+            const int iy = bh_index(d->boundary, sy, 0, H, H);
+            const double xv = X[(long long)iy * in->stride + ix];
+            DX[(long long)iy * W + ix] += (double)w[t] * g;
+            dw[t] += g * xv;
+            if (DXA) DXA[(long long)iy * W + ix] += fabs((double)w[t] * g);
+            if (dw_abs) dw_abs[t] += fabs(g * xv);
+          }
+      }
+  }
+  return 0;
+}
+
+/* ---- bilateral backward (no reference counterpart).  For each output pixel
+ * p the forward of oracle_bilateral_f64 is re-evaluated in double and the
+ * chain rule applied tap by tap in the forward's order: the tap value v read
+ * at X(q) receives a*s*(1 - 2c (v - X(p)) (v - Y(p))), the centre X(p)
+ * receives 2c a SUM s (v - X(p)) (v - Y(p)), and dL/dc collects
+ * -a SUM s (v - X(p))^2 (v - Y(p)), with a = dY(p)/D(p), c = 0.5/sigma_r^2.
+ * Outputs: dx dense [batch][h][w], *dsigma = dL/dc * (-1/sigma_r^3); the
+ * optional dx_abs and *dsigma_abs are the sums of the absolute values of the
+ * terms (the scale of each result's rounding). */
+ORC_EXPORT int oracle_bilateral_grad(const hipacc_bilateral_grad_desc* d, double* dx,
+                                     double* dsigma, double* dx_abs, double* dsigma_abs) {
+  const hipacc_view* in = &d->in;
+  const hipacc_view* gy = &d->grad_out;
+  const int W = in->width, H = in->height, k = d->ksize, r = k / 2, nb = nbatch(in);
+  hipacc_bilateral_desc bd;
+  memset(&bd, 0, sizeof(bd));
+  bd.ksize = d->ksize;
+  bd.sigma_d = d->sigma_d;
+  bd.spatial = d->spatial;
+  float sp[HIPACC_MAX_MASK_DIM * HIPACC_MAX_MASK_DIM];
+  spatial_table(&bd, sp);
+  const double c = 0.5 / ((double)d->sigma_r * d->sigma_r);
+  const double cv = (double)(float)d->const_val;
+  double dc = 0.0, dc_abs = 0.0;
+  memset(dx, 0, sizeof(double) * (size_t)nb * W * H);
+  if (dx_abs) memset(dx_abs, 0, sizeof(double) * (size_t)nb * W * H);
+  for (int b = 0; b < nb; ++b) {
+    const float* X = (const float*)plane(in, b);
+    const float* G = (const float*)plane(gy, b);
+    double* DX = dx + (size_t)b * W * H;
+    double* DXA = dx_abs ? dx_abs + (size_t)b * W * H : NULL;
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < W; ++x) {
+        const double xp = X[(long long)y * in->stride + x];
+        double v[HIPACC_MAX_MASK_DIM * HIPACC_MAX_MASK_DIM], s[HIPACC_MAX_MASK_DIM * HIPACC_MAX_MASK_DIM];
+        long long q[HIPACC_MAX_MASK_DIM * HIPACC_MAX_MASK_DIM];
+        double D = 0.0, N = 0.0;
+        for (int ty = 0; ty < k; ++ty)
+          for (int tx = 0; tx < k; ++tx) {
+            const int t = ty * k + tx;
+            const int sx = x + tx - r, sy = y + ty - r;
+            if (d->boundary == HIPACC_BOUNDARY_CONSTANT && (sx < 0 || sx >= W || sy < 0 || sy >= H)) {
+              q[t] = -1;
+              v[t] = cv;
+            } else {
+              const int ix = bh_index(d->boundary, sx, 0, W, W);
+              const int iy = bh_index(d->boundary, sy, 0, H, H);
+              q[t] = (long long)iy * W + ix;
+              v[t] = X[(long long)iy * in->stride + ix];
+            }
+            s[t] = exp(-c * (v[t] - xp) * (v[t] - xp)) * (double)sp[t];
+            D += s[t];
+            N += s[t] * v[t];
+          }
+        const double Y = N / D;
+        const double a = (double)G[(long long)y * gy->stride + x] / D;
+        double centre = 0.0, centre_abs = 0.0;
+        for (int t = 0; t < k * k; ++t) {
+          const double dv = v[t] - xp, ey = v[t] - Y;
+          if (q[t] >= 0) {
+            const double term = a * s[t] * (1.0 - 2.0 * c * dv * ey);
+            DX[q[t]] += term;
+            if (DXA) DXA[q[t]] += fabs(term);
+          }
+          centre += s[t] * dv * ey;
+          centre_abs += fabs(s[t] * dv * ey);
+          dc -= a * s[t] * dv * dv * ey;
+          dc_abs += fabs(a * s[t] * dv * dv * ey);
+        }
+        DX[(long long)y * W + x] += 2.0 * c * a * centre;
+        if (DXA) DXA[(long long)y * W + x] += fabs(2.0 * c * a) * centre_abs;
+      }
+  }
+  const double sr = d->sigma_r;
+  *dsigma = dc * (-1.0 / (sr * sr * sr));
+  if (dsigma_abs) *dsigma_abs = dc_abs / (sr * sr * sr);
+  return 0;
+}
+
 /* ---- synthetic pixels (same generator as the GPU fill kernel) ---------- */
 ORC_EXPORT int oracle_fill_random(const hipacc_view* v, uint64_t seed) {
   for (int b = 0; b < nbatch(v); ++b) {

@@ -15,6 +15,16 @@ int main(void) {
   F(hipacc_point_desc, out); F(hipacc_point_desc, op); F(hipacc_point_desc, cvt);
   F(hipacc_bilateral_desc, ksize); F(hipacc_bilateral_desc, spatial);
   F(hipacc_bilateral_desc, const_val); F(hipacc_bilateral_desc, cvt);
+  F(hipacc_local_grad_desc, grad_out); F(hipacc_local_grad_desc, grad_in);
+  F(hipacc_local_grad_desc, mask_w); F(hipacc_local_grad_desc, weights_dev);
+  F(hipacc_local_grad_desc, grad_weights_dev); F(hipacc_local_grad_desc, const_val);
+  F(hipacc_local_grad_desc, workspace);
+  F(hipacc_bilateral_grad_desc, norm); F(hipacc_bilateral_grad_desc, grad_in);
+  F(hipacc_bilateral_grad_desc, ksize); F(hipacc_bilateral_grad_desc, spatial);
+  F(hipacc_bilateral_grad_desc, const_val); F(hipacc_bilateral_grad_desc, grad_sigma_r_dev);
+  F(hipacc_bilateral_grad_desc, workspace);
+  printf("\"sizeof.lgrad\": %zu, \"sizeof.bgrad\": %zu,\n", sizeof(hipacc_local_grad_desc),
+         sizeof(hipacc_bilateral_grad_desc));
   printf("\"sizeof.view\": %zu, \"sizeof.local\": %zu, \"sizeof.sep\": %zu, \"sizeof.point\": %zu, \"sizeof.bilat\": %zu\n}\n",
          sizeof(hipacc_view), sizeof(hipacc_local_desc), sizeof(hipacc_separable_desc),
          sizeof(hipacc_point_desc), sizeof(hipacc_bilateral_desc));

@@ -45,6 +45,10 @@ def lib() -> ctypes.CDLL:
         L.oracle_bilateral_f64.argtypes = [POINTER(capi.BilateralDesc), c_void_p]
         L.oracle_fill_random.argtypes = [POINTER(capi.View), c_uint64]
         L.oracle_num_threads.restype = c_int
+        L.oracle_local_grad.argtypes = [POINTER(capi.LocalGradDesc), c_void_p, c_void_p, c_void_p,
+                                        c_void_p]
+        L.oracle_bilateral_grad.argtypes = [POINTER(capi.BilateralGradDesc), c_void_p, c_void_p,
+                                            c_void_p, c_void_p]
         _LIB = L
     return _LIB
 
@@ -204,3 +208,46 @@ def bilateral(src, dst, ksize, sigma_d, sigma_r, *, spatial=None, boundary=capi.
     else:
         lib().oracle_bilateral(ctypes.byref(d))
     del keep
+
+
+# ---- adjoints ----------------------------------------------------------------
+def local_grad(src, grad_out, weights, mask_w, mask_h, *, boundary=capi.CLAMP, const_val=0.0,
+               with_abs=False):
+    """(dx, dw[, dx_abs, dw_abs]) of the float weighted SUM, in double."""
+    d = capi.LocalGradDesc()
+    d.input = hview(src)
+    d.grad_out = hview(grad_out)
+    d.mask_w, d.mask_h = mask_w, mask_h
+    w, d.weights_f32 = _arr(weights, np.float32)
+    d.boundary, d.const_val = boundary, const_val
+    dx = np.zeros(src.shape, np.float64)
+    dw = np.zeros(mask_w * mask_h, np.float64)
+    dxa = np.zeros_like(dx) if with_abs else None
+    dwa = np.zeros_like(dw) if with_abs else None
+    lib().oracle_local_grad(ctypes.byref(d), dx.ctypes.data, dw.ctypes.data,
+                            dxa.ctypes.data if with_abs else None,
+                            dwa.ctypes.data if with_abs else None)
+    del w
+    return (dx, dw, dxa, dwa) if with_abs else (dx, dw)
+
+
+def bilateral_grad(src, grad_out, ksize, sigma_d, sigma_r, *, spatial=None, boundary=capi.MIRROR,
+                   const_val=0.0, with_abs=False):
+    """(dx, dsigma[, dx_abs, dsigma_abs]) of the bilateral filter, in double."""
+    d = capi.BilateralGradDesc()
+    d.input = hview(src)
+    d.grad_out = hview(grad_out)
+    d.ksize, d.sigma_d, d.sigma_r = ksize, sigma_d, sigma_r
+    keep = None
+    if spatial is not None:
+        keep, d.spatial = _arr(spatial, np.float32)
+    d.boundary, d.const_val = boundary, const_val
+    dx = np.zeros(src.shape, np.float64)
+    ds = np.zeros(1, np.float64)
+    dxa = np.zeros_like(dx) if with_abs else None
+    dsa = np.zeros(1, np.float64) if with_abs else None
+    lib().oracle_bilateral_grad(ctypes.byref(d), dx.ctypes.data, ds.ctypes.data,
+                                dxa.ctypes.data if with_abs else None,
+                                dsa.ctypes.data if with_abs else None)
+    del keep
+    return (dx, ds[0], dxa, dsa[0]) if with_abs else (dx, ds[0])

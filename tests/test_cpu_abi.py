@@ -37,13 +37,16 @@ def test_struct_layout_matches_ctypes(tmp_path):
                                     text=True).stdout)
     py = {"hipacc_view": capi.View, "hipacc_local_desc": capi.LocalDesc,
           "hipacc_separable_desc": capi.SeparableDesc, "hipacc_point_desc": capi.PointDesc,
-          "hipacc_bilateral_desc": capi.BilateralDesc}
+          "hipacc_bilateral_desc": capi.BilateralDesc,
+          "hipacc_local_grad_desc": capi.LocalGradDesc,
+          "hipacc_bilateral_grad_desc": capi.BilateralGradDesc}
     rename = {"in": "input", "out": "output"}
     for key, off in lay.items():
         t, m = key.split(".")
         if t == "sizeof":
             cls = {"view": capi.View, "local": capi.LocalDesc, "sep": capi.SeparableDesc,
-                   "point": capi.PointDesc, "bilat": capi.BilateralDesc}[m]
+                   "point": capi.PointDesc, "bilat": capi.BilateralDesc,
+                   "lgrad": capi.LocalGradDesc, "bgrad": capi.BilateralGradDesc}[m]
             assert ctypes.sizeof(cls) == off, key
         else:
             assert getattr(py[t], rename.get(m, m)).offset == off, key
