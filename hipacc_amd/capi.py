@@ -77,6 +77,28 @@ class BilateralGradDesc(ctypes.Structure):
                 ("grad_sigma_r_dev", c_void_p), ("workspace", c_void_p)]
 
 
+INTERP_NN, INTERP_LF = 1, 2
+RESAMPLE_COPY, RESAMPLE_SUB, RESAMPLE_ADD = 0, 1, 2
+MAX_LEVELS = 16
+
+
+class ResampleDesc(ctypes.Structure):
+    _fields_ = [("input", View), ("in2", View), ("output", View), ("interp", c_int32),
+                ("op", c_int32), ("boundary", c_int32), ("const_val", c_double)]
+
+
+class PyramidImage(ctypes.Structure):
+    _fields_ = [("width", c_int32), ("height", c_int32),
+                ("gauss", c_void_p * MAX_LEVELS), ("gauss_stride", c_int32 * MAX_LEVELS),
+                ("lap", c_void_p * MAX_LEVELS), ("lap_stride", c_int32 * MAX_LEVELS)]
+
+
+class PyramidDesc(ctypes.Structure):
+    _fields_ = [("num_images", c_int32), ("levels", c_int32),
+                ("images", POINTER(PyramidImage)), ("mask", c_void_p), ("boundary", c_int32),
+                ("fp_contract", c_int32), ("tile_rows", c_int32)]
+
+
 class TuneEntry(ctypes.Structure):
     _fields_ = [("kind", c_int32), ("kx", c_int32), ("ky", c_int32), ("in_dtype", c_int32),
                 ("out_dtype", c_int32), ("acc_dtype", c_int32), ("boundary", c_int32),
@@ -103,6 +125,12 @@ SIGNATURES = {
     "hipacc_hip_histogram": (c_int32, [POINTER(View), c_int32, c_float, c_void_p, c_void_p]),
     "hipacc_hip_bilateral": (c_int32, [POINTER(BilateralDesc), c_void_p]),
     "hipacc_hip_grad_workspace_bytes": (c_size_t, []),
+    "hipacc_hip_resample": (c_int32, [POINTER(ResampleDesc), c_void_p]),
+    "hipacc_hip_pyramid_plan_create": (c_int32, [POINTER(PyramidDesc), c_void_p,
+                                                 POINTER(c_void_p)]),
+    "hipacc_hip_pyramid_run": (c_int32, [c_void_p, c_void_p]),
+    "hipacc_hip_pyramid_run_direct": (c_int32, [c_void_p, c_void_p]),
+    "hipacc_hip_pyramid_plan_destroy": (c_int32, [c_void_p]),
     "hipacc_hip_local_grad": (c_int32, [POINTER(LocalGradDesc), c_void_p]),
     "hipacc_hip_bilateral_train": (c_int32, [POINTER(BilateralGradDesc), c_void_p]),
     "hipacc_hip_bilateral_grad": (c_int32, [POINTER(BilateralGradDesc), c_void_p]),

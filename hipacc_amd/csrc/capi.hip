@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "hh_grad.hpp"
+#include "hh_pyramid.hpp"
 #include "hh_tile.hpp"
 
 namespace hh {
@@ -868,6 +869,58 @@ int hipacc_hip_bilateral_grad(const hipacc_bilateral_grad_desc* d, void* stream)
   hipStream_t s = static_cast<hipStream_t>(stream);
   Timer tm(s);
   return tm.finish(launch_bilateral_grad(a, ds ? d->grad_sigma_r_dev : nullptr, d->sigma_r, s));
+}
+
+int hipacc_hip_resample(const hipacc_resample_desc* d, void* stream) {
+  if (!d) return set_error(HIPACC_ERR_INVALID_ARG, "resample: desc is NULL");
+  int st = check_view(d->in, "in");
+  if (st) return st;
+  if ((st = check_view(d->out, "out"))) return st;
+  if (d->in.dtype != HIPACC_F32 || d->out.dtype != HIPACC_F32)
+    return set_error(HIPACC_ERR_UNSUPPORTED, "resample: F32 images only");
+  if (batch_of(d->in) != 1 || batch_of(d->out) != 1)
+    return set_error(HIPACC_ERR_UNSUPPORTED, "resample: single images only");
+  if (d->interp != HIPACC_INTERP_NN && d->interp != HIPACC_INTERP_LF)
+    return set_error(HIPACC_ERR_INVALID_ARG, "resample: interp %d", d->interp);
+  if (d->op < HIPACC_RESAMPLE_COPY || d->op > HIPACC_RESAMPLE_ADD)
+    return set_error(HIPACC_ERR_INVALID_ARG, "resample: op %d", d->op);
+  if (d->boundary < HIPACC_BOUNDARY_UNDEFINED || d->boundary > HIPACC_BOUNDARY_CONSTANT)
+    return set_error(HIPACC_ERR_INVALID_ARG, "resample: boundary %d", d->boundary);
+  if (d->op != HIPACC_RESAMPLE_COPY) {
+    if ((st = check_view(d->in2, "in2"))) return st;
+    if (d->in2.dtype != HIPACC_F32 || d->in2.width != d->out.width ||
+        d->in2.height != d->out.height || batch_of(d->in2) != 1)
+      return set_error(HIPACC_ERR_INVALID_ARG, "resample: in2 must be F32 of out's size");
+  }
+  ResampleArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.in = static_cast<const float*>(d->in.data);
+  a.in2 = static_cast<const float*>(d->in2.data);
+  a.out = static_cast<float*>(d->out.data);
+  a.in_stride = d->in.stride;
+  a.in2_stride = d->in2.stride;
+  a.out_stride = d->out.stride;
+  a.in_imgw = d->in.img_width;
+  a.in_imgh = d->in.img_height;
+  a.lo_x = d->in.offset_x;
+  a.lo_y = d->in.offset_y;
+  a.hi_x = d->in.offset_x + d->in.width;
+  a.hi_y = d->in.offset_y + d->in.height;
+  a.in2_ox = d->in2.offset_x;
+  a.in2_oy = d->in2.offset_y;
+  a.out_ox = d->out.offset_x;
+  a.out_oy = d->out.offset_y;
+  a.w = d->out.width;
+  a.h = d->out.height;
+  a.sx = (float)d->in.width / (float)d->out.width;
+  a.sy = (float)d->in.height / (float)d->out.height;
+  a.interp = d->interp;
+  a.op = d->op;
+  a.boundary = d->boundary;
+  a.cval = (float)d->const_val;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Timer tm(s);
+  return tm.finish(launch_resample(a, s));
 }
 
 int hipacc_hip_alloc2d(void** ptr, int32_t* stride_elems, int32_t width, int32_t height,

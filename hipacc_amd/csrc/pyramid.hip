@@ -1,0 +1,338 @@
+// pyramid.hip -- resampling point operators (Interpolate::NN / LF) and the
+// batched Gaussian / Laplacian pyramid: one persistent launch per level over
+// a work list of 64-column tiles of every image, the launches captured once
+// into a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "hh_pyramid.hpp"
+
+namespace hh {
+
+// ---------------------------------------------------------------------------
+// Resampling point operator
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float rs_read(const ResampleArgs& a, int x, int y) {
+  if (a.boundary == HIPACC_BOUNDARY_CONSTANT) {
+    if (x < a.lo_x || x >= a.hi_x || y < a.lo_y || y >= a.hi_y) return a.cval;
+  } else {
+    x = bh_map(a.boundary, x, a.lo_x, a.hi_x, a.in_imgw);
+    y = bh_map(a.boundary, y, a.lo_y, a.hi_y, a.in_imgh);
+  }
+  return a.in[(long long)y * a.in_stride + x];
+}
+
+__global__ __launch_bounds__(256) void resample_kernel(const ResampleArgs a) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= a.w || y >= a.h) return;
+  const float mx = interp_map(a.sx, x, a.lo_x);
+  const float my = interp_map(a.sy, y, a.lo_y);
+  float v;
+  if (a.interp == HIPACC_INTERP_NN) {
+    v = rs_read(a, (int)mx, (int)my);
+  } else {
+    int xi, yi;
+    float fx, fy;
+    interp_lf(mx, xi, fx);
+    interp_lf(my, yi, fy);
+    v = interp_bilinear(fx, fy, rs_read(a, xi, yi), rs_read(a, xi + 1, yi),
+                        rs_read(a, xi, yi + 1), rs_read(a, xi + 1, yi + 1));
+  }
+  if (a.op != HIPACC_RESAMPLE_COPY) {
+    const float b = a.in2[(long long)(y + a.in2_oy) * a.in2_stride + x + a.in2_ox];
+    v = a.op == HIPACC_RESAMPLE_SUB ? b - v : v + b;
+  }
+  a.out[(long long)(y + a.out_oy) * a.out_stride + x + a.out_ox] = v;
+}
+
+int launch_resample(const ResampleArgs& a, hipStream_t s) {
+  dim3 grid(ceil_div(a.w, 64), ceil_div(a.h, 4));
+  hipLaunchKernelGGL(resample_kernel, grid, dim3(256), 0, s, a);
+  return hip_status(hipGetLastError(), "resample_kernel launch");
+}
+
+// ---------------------------------------------------------------------------
+// Pyramid level kernel
+// ---------------------------------------------------------------------------
+// G[l] read with the convolution's boundary mode over the whole image.
+__device__ __forceinline__ float pyr_read(const float* g, int stride, int w, int h, int mode,
+                                          int x, int y) {
+  if (x < 0 || x >= w || y < 0 || y >= h) {
+    if (mode == HIPACC_BOUNDARY_CONSTANT) return 0.0f;
+    x = bh_map(mode, x, 0, w, w);
+    y = bh_map(mode, y, 0, h, h);
+  }
+  return g[(long long)y * stride + x];
+}
+
+// For each tile (TW x TH pixels of level l): the G[l+1] values the tile's
+// LF reads and the ones it owns (columns x0/2.., rows y0/2..) are computed
+// into LDS -- the 5x5 convolution of G[l] evaluated only at the NN sample
+// points -- the owned ones are stored, then L[l] = G[l] - LF(G[l+1]).
+template <int TH, bool FMA>
+__global__ __launch_bounds__(256) void pyramid_level_kernel(const PyrLevelArgs a) {
+  constexpr int TW = 64;
+  constexpr int UW = TW / 2 + 4, UH = TH / 2 + 4;   // G[l+1] region capacity
+  __shared__ float g1s[UH * UW];
+  const int l = a.level;
+
+  for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+    const PyrTile tile = a.tiles[t];
+    const PyrImage& im = a.imgs[tile.img];
+    const int W = im.w[l], H = im.h[l], W1 = im.w[l + 1], H1 = im.h[l + 1];
+    const float* G = im.g[l];
+    float* G1 = im.g[l + 1];
+    float* L = im.lap[l];
+    const int gs = im.gs[l], g1st = im.gs[l + 1], ls = im.ls[l];
+    const int x0 = tile.x0, y0 = tile.y0;
+    const int xe = min(x0 + TW, W) - 1, ye = min(y0 + TH, H) - 1;
+    // G[l+1] region: LF taps of the tile plus the owned block.
+    const float su = (float)W1 / (float)W, sv = (float)H1 / (float)H;
+    int ux0, ux1, uy0, uy1;
+    float f;
+    interp_lf(interp_map(su, x0), ux0, f);
+    interp_lf(interp_map(su, xe), ux1, f);
+    interp_lf(interp_map(sv, y0), uy0, f);
+    interp_lf(interp_map(sv, ye), uy1, f);
+    ux1 = min(ux1 + 1, W1 - 1);
+    uy1 = min(uy1 + 1, H1 - 1);
+    const int ox0 = x0 / 2, ox1 = min(x0 / 2 + TW / 2, W1) - 1;
+    const int oy0 = y0 / 2, oy1 = min(y0 / 2 + TH / 2, H1) - 1;
+    ux0 = min(ux0, ox0);
+    uy0 = min(uy0, oy0);
+    ux1 = max(ux1, ox1);
+    uy1 = max(uy1, oy1);
+    const int uw = ux1 - ux0 + 1, uh = uy1 - uy0 + 1;
+
+    __syncthreads();  // previous tile's LDS reads are done
+    if (uw <= UW && uh <= UH) {
+      const float sd = (float)W / (float)W1, td = (float)H / (float)H1;
+      for (int i = threadIdx.x; i < uw * uh; i += 256) {
+        const int yy = i / uw, xx = i - yy * uw;
+        const int xu = ux0 + xx, yu = uy0 + yy;
+        const int nx = (int)interp_map(sd, xu), ny = (int)interp_map(td, yu);
+        float acc = 0.0f;
+#pragma unroll
+        for (int ty = 0; ty < 5; ++ty)
+#pragma unroll
+          for (int tx = 0; tx < 5; ++tx) {
+            const float v = pyr_read(G, gs, W, H, a.boundary, nx + tx - 2, ny + ty - 2);
+            const float wv = a.w[ty * 5 + tx];
+            if (ty == 0 && tx == 0)
+              acc = mul_first(wv, v);
+            else
+              acc = madd<FMA>(wv, v, acc);
+          }
+        g1s[yy * UW + xx] = acc;
+        if (xu >= ox0 && xu <= ox1 && yu >= oy0 && yu <= oy1)
+          G1[(long long)yu * g1st + xu] = acc;
+      }
+    }
+    __syncthreads();
+    if (uw > UW || uh > UH) continue;   // never for the level sizes of a pyramid
+
+    for (int i = threadIdx.x; i < TW * TH; i += 256) {
+      const int r = i / TW, c = i - r * TW;
+      const int x = x0 + c, y = y0 + r;
+      if (x >= W || y >= H) continue;
+      int xi, yi;
+      float fx, fy;
+      interp_lf(interp_map(su, x), xi, fx);
+      interp_lf(interp_map(sv, y), yi, fy);
+      const int xa = xi - ux0, xb = min(xi + 1, W1 - 1) - ux0;
+      const int ya = yi - uy0, yb = min(yi + 1, H1 - 1) - uy0;
+      const float lf = interp_bilinear(fx, fy, g1s[ya * UW + xa], g1s[ya * UW + xb],
+                                       g1s[yb * UW + xa], g1s[yb * UW + xb]);
+      L[(long long)y * ls + x] = G[(long long)y * gs + x] - lf;
+    }
+  }
+}
+
+}  // namespace hh
+
+// ---------------------------------------------------------------------------
+// Plans
+// ---------------------------------------------------------------------------
+struct hipacc_pyramid_plan {
+  int levels = 0, nimg = 0, tile_rows = 16, fma = 1, boundary = HIPACC_BOUNDARY_CLAMP;
+  float w[25];
+  hh::PyrImage* d_imgs = nullptr;
+  hh::PyrTile* d_tiles = nullptr;
+  std::vector<int> off, cnt;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace hh {
+
+template <int TH, bool FMA>
+static int launch_level_t(const PyrLevelArgs& a, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL((pyramid_level_kernel<TH, FMA>), dim3(nblocks), dim3(256), 0, s, a);
+  return hip_status(hipGetLastError(), "pyramid_level_kernel launch");
+}
+
+static int plan_launches(hipacc_pyramid_plan* p, hipStream_t s) {
+  for (int l = 0; l + 1 < p->levels; ++l) {
+    if (p->cnt[l] == 0) continue;
+    PyrLevelArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.imgs = p->d_imgs;
+    a.tiles = p->d_tiles + p->off[l];
+    a.ntiles = p->cnt[l];
+    a.level = l;
+    a.boundary = p->boundary;
+    std::memcpy(a.w, p->w, sizeof(a.w));
+    const int nb = a.ntiles < 2048 ? a.ntiles : 2048;
+    int st;
+    if (p->fma) {
+      st = p->tile_rows == 8    ? launch_level_t<8, true>(a, nb, s)
+           : p->tile_rows == 32 ? launch_level_t<32, true>(a, nb, s)
+                                : launch_level_t<16, true>(a, nb, s);
+    } else {
+      st = p->tile_rows == 8    ? launch_level_t<8, false>(a, nb, s)
+           : p->tile_rows == 32 ? launch_level_t<32, false>(a, nb, s)
+                                : launch_level_t<16, false>(a, nb, s);
+    }
+    if (st) return st;
+  }
+  return HIPACC_OK;
+}
+
+static void plan_free(hipacc_pyramid_plan* p) {
+  if (!p) return;
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->d_imgs) (void)hipFree(p->d_imgs);
+  if (p->d_tiles) (void)hipFree(p->d_tiles);
+  delete p;
+}
+
+}  // namespace hh
+
+using namespace hh;
+
+extern "C" {
+
+int hipacc_hip_pyramid_plan_create(const hipacc_pyramid_desc* d, void* stream,
+                                   hipacc_pyramid_plan** out) {
+  if (!d || !out) return set_error(HIPACC_ERR_INVALID_ARG, "pyramid: NULL argument");
+  *out = nullptr;
+  if (d->num_images < 1 || !d->images)
+    return set_error(HIPACC_ERR_INVALID_ARG, "pyramid: no images");
+  if (d->levels < 2 || d->levels > HIPACC_MAX_LEVELS)
+    return set_error(HIPACC_ERR_INVALID_ARG, "pyramid: levels %d (2..%d)", d->levels,
+                     HIPACC_MAX_LEVELS);
+  if (d->boundary < HIPACC_BOUNDARY_UNDEFINED || d->boundary > HIPACC_BOUNDARY_CONSTANT)
+    return set_error(HIPACC_ERR_INVALID_ARG, "pyramid: boundary %d", d->boundary);
+  if (d->tile_rows != 0 && d->tile_rows != 8 && d->tile_rows != 16 && d->tile_rows != 32)
+    return set_error(HIPACC_ERR_INVALID_ARG, "pyramid: tile_rows %d (0, 8, 16, 32)", d->tile_rows);
+  hipacc_pyramid_plan* p = new hipacc_pyramid_plan;
+  p->levels = d->levels;
+  p->nimg = d->num_images;
+  p->tile_rows = d->tile_rows ? d->tile_rows : 16;
+  p->fma = d->fp_contract != 0;
+  p->boundary = d->boundary;
+  static const float b5[5] = {1.0f, 4.0f, 6.0f, 4.0f, 1.0f};
+  for (int t = 0; t < 25; ++t) p->w[t] = d->mask ? d->mask[t] : b5[t / 5] * b5[t % 5] / 256.0f;
+
+  std::vector<PyrImage> imgs(d->num_images);
+  p->off.assign(d->levels, 0);
+  p->cnt.assign(d->levels, 0);
+  std::vector<std::vector<PyrTile>> lists(d->levels);
+  for (int i = 0; i < d->num_images; ++i) {
+    const hipacc_pyramid_image& s = d->images[i];
+    PyrImage& m = imgs[i];
+    std::memset(&m, 0, sizeof(m));
+    int w = s.width, h = s.height;
+    for (int l = 0; l < d->levels; ++l) {
+      if (w < 1 || h < 1) {
+        plan_free(p);
+        return set_error(HIPACC_ERR_INVALID_ARG, "pyramid: image %d too small for %d levels", i,
+                         d->levels);
+      }
+      if (!s.gauss[l] || s.gauss_stride[l] < w || (l + 1 < d->levels && (!s.lap[l] || s.lap_stride[l] < w))) {
+        plan_free(p);
+        return set_error(HIPACC_ERR_INVALID_ARG, "pyramid: image %d level %d buffers", i, l);
+      }
+      m.w[l] = w;
+      m.h[l] = h;
+      m.g[l] = s.gauss[l];
+      m.gs[l] = s.gauss_stride[l];
+      if (l + 1 < d->levels) {
+        m.lap[l] = s.lap[l];
+        m.ls[l] = s.lap_stride[l];
+        for (int y0 = 0; y0 < h; y0 += p->tile_rows)
+          for (int x0 = 0; x0 < w; x0 += 64) lists[l].push_back(PyrTile{i, x0, y0});
+      }
+      w /= 2;
+      h /= 2;
+    }
+  }
+  std::vector<PyrTile> all;
+  for (int l = 0; l < d->levels; ++l) {
+    p->off[l] = (int)all.size();
+    p->cnt[l] = (int)lists[l].size();
+    all.insert(all.end(), lists[l].begin(), lists[l].end());
+  }
+  hipError_t e = hipMalloc(&p->d_imgs, sizeof(PyrImage) * imgs.size());
+  if (e == hipSuccess) e = hipMalloc(&p->d_tiles, sizeof(PyrTile) * (all.empty() ? 1 : all.size()));
+  hipStream_t us = static_cast<hipStream_t>(stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(p->d_imgs, imgs.data(), sizeof(PyrImage) * imgs.size(),
+                       hipMemcpyHostToDevice, us);
+  if (e == hipSuccess && !all.empty())
+    e = hipMemcpyAsync(p->d_tiles, all.data(), sizeof(PyrTile) * all.size(),
+                       hipMemcpyHostToDevice, us);
+  if (e == hipSuccess) e = hipStreamSynchronize(us);
+  if (e != hipSuccess) {
+    plan_free(p);
+    return hip_status(e, "pyramid: work list upload");
+  }
+  // Capture the level launches once on a private stream.
+  hipStream_t cs = nullptr;
+  e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    plan_free(p);
+    return hip_status(e, "hipStreamCreateWithFlags");
+  }
+  int st = HIPACC_OK;
+  e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    st = plan_launches(p, cs);
+    hipError_t e2 = hipStreamEndCapture(cs, &p->graph);
+    if (st == HIPACC_OK && e2 != hipSuccess) st = hip_status(e2, "hipStreamEndCapture");
+  } else {
+    st = hip_status(e, "hipStreamBeginCapture");
+  }
+  if (st == HIPACC_OK) {
+    e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) st = hip_status(e, "hipGraphInstantiate");
+  }
+  (void)hipStreamDestroy(cs);
+  if (st) {
+    plan_free(p);
+    return st;
+  }
+  *out = p;
+  return HIPACC_OK;
+}
+
+int hipacc_hip_pyramid_run(hipacc_pyramid_plan* p, void* stream) {
+  if (!p || !p->exec) return set_error(HIPACC_ERR_INVALID_ARG, "pyramid_run: no plan");
+  return hip_status(hipGraphLaunch(p->exec, static_cast<hipStream_t>(stream)), "hipGraphLaunch");
+}
+
+int hipacc_hip_pyramid_run_direct(hipacc_pyramid_plan* p, void* stream) {
+  if (!p) return set_error(HIPACC_ERR_INVALID_ARG, "pyramid_run_direct: no plan");
+  return plan_launches(p, static_cast<hipStream_t>(stream));
+}
+
+int hipacc_hip_pyramid_plan_destroy(hipacc_pyramid_plan* p) {
+  plan_free(p);
+  return HIPACC_OK;
+}
+
+}  // extern "C"

@@ -327,6 +327,88 @@ def prepare_bilateral_grad(src: torch.Tensor, dst: torch.Tensor, norm: torch.Ten
     return Call("hipacc_hip_bilateral_grad", d, keep)
 
 
+def prepare_resample(src: torch.Tensor, dst: torch.Tensor, interp: int, *,
+                     op: int = capi.RESAMPLE_COPY, src2: Optional[torch.Tensor] = None,
+                     boundary: int = capi.CLAMP, const_val: float = 0.0,
+                     src_window: Window = None, dst_window: Window = None) -> Call:
+    """Point operator reading `src` through Interpolate::NN / LF; op SUB
+    gives src2 - v, ADD v + src2."""
+    d = capi.ResampleDesc()
+    d.input = view(src, src_window)
+    if src2 is not None:
+        d.in2 = view(src2)
+    d.output = view(dst, dst_window)
+    d.interp, d.op, d.boundary, d.const_val = interp, op, boundary, const_val
+    return Call("hipacc_hip_resample", d, [src2])
+
+
+def resample(*args, **kw) -> None:
+    prepare_resample(*args, **kw).launch()
+
+
+def pyramid_shapes(h: int, w: int, levels: int):
+    """Level sizes of a pyramid: halved with truncation."""
+    return [(h >> l, w >> l) for l in range(levels)]
+
+
+class PyramidPlan:
+    """Batched Gaussian / Laplacian pyramids of many images.
+
+    gauss[i][l] / lap[i][l] are float32 (H_l, W_l) device tensors (gauss[i][0]
+    is image i's input; lap[i] has levels-1 entries).  ``run()`` replays the
+    captured hipGraph, ``run_direct()`` issues the same launches without it."""
+
+    def __init__(self, gauss, lap, *, mask=None, boundary: int = capi.CLAMP,
+                 fp_contract: bool = True, tile_rows: int = 0):
+        levels = len(gauss[0])
+        n = len(gauss)
+        self.images = (capi.PyramidImage * n)()
+        for i in range(n):
+            im = self.images[i]
+            im.height, im.width = gauss[i][0].shape
+            for l in range(levels):
+                g = gauss[i][l]
+                if g.dtype != torch.float32 or not g.is_cuda or g.stride(-1) != 1:
+                    raise ValueError("pyramid levels must be float32 device tensors with unit "
+                                     "column stride")
+                im.gauss[l] = g.data_ptr()
+                im.gauss_stride[l] = g.stride(0)
+                if l < levels - 1:
+                    im.lap[l] = lap[i][l].data_ptr()
+                    im.lap_stride[l] = lap[i][l].stride(0)
+        d = capi.PyramidDesc()
+        d.num_images, d.levels = n, levels
+        d.images = self.images
+        self._mask = None
+        if mask is not None:
+            self._mask, d.mask = _arr(mask, np.float32)
+        d.boundary, d.fp_contract, d.tile_rows = boundary, int(fp_contract), tile_rows
+        self._keep = (gauss, lap)
+        self.plan = ctypes.c_void_p()
+        capi.check(capi.lib().hipacc_hip_pyramid_plan_create(ctypes.byref(d), _stream(),
+                                                              ctypes.byref(self.plan)),
+                   "hipacc_hip_pyramid_plan_create")
+
+    def run(self, stream: Optional[int] = None) -> None:
+        capi.check(capi.lib().hipacc_hip_pyramid_run(self.plan, _stream() if stream is None
+                                                     else stream), "hipacc_hip_pyramid_run")
+
+    def run_direct(self, stream: Optional[int] = None) -> None:
+        capi.check(capi.lib().hipacc_hip_pyramid_run_direct(
+            self.plan, _stream() if stream is None else stream), "hipacc_hip_pyramid_run_direct")
+
+    def close(self) -> None:
+        if self.plan:
+            capi.lib().hipacc_hip_pyramid_plan_destroy(self.plan)
+            self.plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def fill_random(t: torch.Tensor, seed: int, window: Window = None) -> None:
     v = view(t, window)
     capi.check(capi.lib().hipacc_hip_fill_random(ctypes.byref(v), seed, _stream()),

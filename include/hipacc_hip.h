@@ -324,6 +324,82 @@ typedef struct hipacc_bilateral_grad_desc {
 int hipacc_hip_bilateral_train(const hipacc_bilateral_grad_desc* desc, void* stream);
 int hipacc_hip_bilateral_grad(const hipacc_bilateral_grad_desc* desc, void* stream);
 
+/* ---- resampling point operators (Interpolate::NN / LF) ---------------- */
+/* Per pixel (x, y) of the iteration space `out`, the accessor `in` (an F32
+ * window of w_in x h_in) is read at the mapped position of Interpolation::
+ * interpolate (dsl/image.hpp:390-422), in float:
+ *     s = w_in / (float)out.width;  xm = s/2 + s*x;   (same for y)
+ *   NN: v = in_bh((int)xm, (int)ym)
+ *   LF: xb = max(xm - 0.5f, 0), xi = (int)xb, fx = xb - xi (same for y)
+ *       v = (1-fx)(1-fy) in_bh(xi,yi) + fx(1-fy) in_bh(xi+1,yi)
+ *         + (1-fx) fy in_bh(xi,yi+1) + fx fy in_bh(xi+1,yi+1)
+ *       (products and sums left to right, no contraction)
+ * and combined with `in2` (F32, read at (x, y) of its window, no boundary):
+ *   COPY out = v     SUB out = in2 - v     ADD out = v + in2
+ * These are the Subsample, DifferenceOfGaussian and Restore kernels of the
+ * pyramid sample (samples-public/5_Other/Gaussian_Laplacian_Pyramid/src/
+ * main.cpp:67-117).  in_bh uses `boundary` over the accessor window.        */
+typedef enum { HIPACC_INTERP_NN = 1, HIPACC_INTERP_LF = 2 } hipacc_interp;
+typedef enum { HIPACC_RESAMPLE_COPY = 0, HIPACC_RESAMPLE_SUB = 1,
+               HIPACC_RESAMPLE_ADD = 2 } hipacc_resample_op;
+typedef struct hipacc_resample_desc {
+  hipacc_view in;            /* accessor read through the interpolation     */
+  hipacc_view in2;           /* second operand (SUB / ADD), out's size      */
+  hipacc_view out;           /* iteration space                             */
+  int32_t interp;            /* hipacc_interp                               */
+  int32_t op;                /* hipacc_resample_op                          */
+  int32_t boundary;
+  double  const_val;
+} hipacc_resample_desc;
+int hipacc_hip_resample(const hipacc_resample_desc* desc, void* stream);
+
+/* ---- batched Gaussian / Laplacian pyramids ---------------------------- */
+/* Level sizes halve with truncation (dsl/pyramid.hpp:103-114).  For level
+ * l -> l+1 of every image (the pyramid sample's traversal body,
+ * samples-public/5_Other/Gaussian_Laplacian_Pyramid/src/main.cpp:196-223):
+ *   G[l+1] = NN-subsample( conv5x5(G[l], mask, boundary) )
+ *   L[l]   = G[l] - LF-upsample(G[l+1])          (LF accessor: CLAMP)
+ * with the convolution as hipacc_hip_local (float SUM, row-major taps,
+ * fp_contract as there).  L[levels-1] is G[levels-1] itself (not written).
+ * A plan holds, for all images, one work list of 64x16 tiles per level and
+ * executes each level as one persistent launch over that list (the blur is
+ * evaluated only at the subsampled points, the upsampled G[l+1] comes from
+ * LDS); the plan's launches are captured once into a hipGraph and replayed
+ * by hipacc_hip_pyramid_run.  Results equal the three-kernels-per-level
+ * sequence of the sample, run per image with hipacc_hip_local and
+ * hipacc_hip_resample, bit for bit.
+ * Replaces per-level kernel launches driven by HipaccPyramidTraversor
+ * (runtime/hipacc_base_standalone.hpp:85-254).                              */
+#define HIPACC_MAX_LEVELS 16
+typedef struct hipacc_pyramid_image {
+  int32_t width, height;                 /* level 0                          */
+  float*  gauss[HIPACC_MAX_LEVELS];      /* G[l], device; G[0] is the input  */
+  int32_t gauss_stride[HIPACC_MAX_LEVELS];
+  float*  lap[HIPACC_MAX_LEVELS];        /* L[l], device, l < levels-1       */
+  int32_t lap_stride[HIPACC_MAX_LEVELS];
+} hipacc_pyramid_image;
+typedef struct hipacc_pyramid_desc {
+  int32_t num_images;
+  int32_t levels;                        /* 2..HIPACC_MAX_LEVELS             */
+  const hipacc_pyramid_image* images;    /* host array of num_images         */
+  const float* mask;                     /* host 5x5, NULL: (1 4 6 4 1)^T (1 4 6 4 1) / 256 */
+  int32_t boundary;                      /* of the convolution (CONSTANT: 0) */
+  int32_t fp_contract;
+  int32_t tile_rows;                     /* rows of L per tile: 0 = tuned/default (16), 8, 16, 32 */
+} hipacc_pyramid_desc;
+typedef struct hipacc_pyramid_plan hipacc_pyramid_plan;
+/* Builds the work lists (device memory, uploaded on `stream`, which is
+ * synchronised) and captures the level launches into a graph.  The image
+ * buffers must stay allocated while the plan lives; their contents may
+ * change between runs.                                                      */
+int hipacc_hip_pyramid_plan_create(const hipacc_pyramid_desc* desc, void* stream,
+                                   hipacc_pyramid_plan** plan);
+/* Replays the captured graph on `stream` (asynchronous).                   */
+int hipacc_hip_pyramid_run(hipacc_pyramid_plan* plan, void* stream);
+/* The same launches issued directly, without the graph.                    */
+int hipacc_hip_pyramid_run_direct(hipacc_pyramid_plan* plan, void* stream);
+int hipacc_hip_pyramid_plan_destroy(hipacc_pyramid_plan* plan);
+
 /* ---- kernel variants and the autotuner ------------------------------- */
 /* A local or separable operator runs one of three kernel variants:
  *   1 tile    LDS-staged tile + halo; param = output rows per wave (2/4/8)

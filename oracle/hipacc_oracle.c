@@ -625,6 +625,48 @@ ORC_EXPORT int oracle_bilateral_f64(const hipacc_bilateral_desc* d, double* out_
   return 0;
 }
 
+/* ---- resampling: Interpolation::interpolate (dsl/image.hpp:390-422) for
+ * Interpolate::NN / LF, in float with the DSL's evaluation order (this file
+ * is compiled without contraction), reading through the accessor's boundary
+ * handling; then COPY / SUB (in2 - v) / ADD (v + in2). -------------------- */
+ORC_EXPORT int oracle_resample(const hipacc_resample_desc* d) {
+  const hipacc_view* in = &d->in;
+  const hipacc_view* out = &d->out;
+  const float stride_x = in->width / (float)out->width;
+  const float stride_y = in->height / (float)out->height;
+  const float* ip = (const float*)in->data;
+  const float cf = (float)d->const_val;
+  for (int y = 0; y < out->height; ++y)
+    for (int x = 0; x < out->width; ++x) {
+      const float x_mapped = in->offset_x + stride_x / 2 + stride_x * x;
+      const float y_mapped = in->offset_y + stride_y / 2 + stride_y * y;
+      float v;
+      long long idx = 0;
+      if (d->interp == HIPACC_INTERP_NN) {
+        v = bh_tap(in, d->boundary, (int)x_mapped, (int)y_mapped, &idx) ? ip[idx] : cf;
+      } else {
+        float xb = x_mapped - 0.5f, yb = y_mapped - 0.5f;
+        if (xb < 0.0f) xb = 0.0f;
+        if (yb < 0.0f) yb = 0.0f;
+        const int xi = (int)xb, yi = (int)yb;
+        const float xf = xb - xi, yf = yb - yi;
+        const float p00 = bh_tap(in, d->boundary, xi, yi, &idx) ? ip[idx] : cf;
+        const float p10 = bh_tap(in, d->boundary, xi + 1, yi, &idx) ? ip[idx] : cf;
+        const float p01 = bh_tap(in, d->boundary, xi, yi + 1, &idx) ? ip[idx] : cf;
+        const float p11 = bh_tap(in, d->boundary, xi + 1, yi + 1, &idx) ? ip[idx] : cf;
+        v = (1.0f - xf) * (1.0f - yf) * p00 + xf * (1.0f - yf) * p10 +
+            (1.0f - xf) * yf * p01 + xf * yf * p11;
+      }
+      if (d->op != HIPACC_RESAMPLE_COPY) {
+        const float b = ((const float*)d->in2.data)[(long long)(y + d->in2.offset_y) * d->in2.stride +
+                                                    x + d->in2.offset_x];
+        v = d->op == HIPACC_RESAMPLE_SUB ? b - v : v + b;
+      }
+      ((float*)out->data)[(long long)(y + out->offset_y) * out->stride + x + out->offset_x] = v;
+    }
+  return 0;
+}
+
 /* ---- adjoint of the float weighted SUM (no reference counterpart: the
  * reference has no gradients).  Scatter form, the literal transpose of the
  * forward's index map (dsl/kernel.hpp:241-267 with the boundary maps of

@@ -45,6 +45,7 @@ def lib() -> ctypes.CDLL:
         L.oracle_bilateral_f64.argtypes = [POINTER(capi.BilateralDesc), c_void_p]
         L.oracle_fill_random.argtypes = [POINTER(capi.View), c_uint64]
         L.oracle_num_threads.restype = c_int
+        L.oracle_resample.argtypes = [POINTER(capi.ResampleDesc)]
         L.oracle_local_grad.argtypes = [POINTER(capi.LocalGradDesc), c_void_p, c_void_p, c_void_p,
                                         c_void_p]
         L.oracle_bilateral_grad.argtypes = [POINTER(capi.BilateralGradDesc), c_void_p, c_void_p,
@@ -251,3 +252,37 @@ def bilateral_grad(src, grad_out, ksize, sigma_d, sigma_r, *, spatial=None, boun
                                 dsa.ctypes.data if with_abs else None)
     del keep
     return (dx, ds[0], dxa, dsa[0]) if with_abs else (dx, ds[0])
+
+
+# ---- resampling and pyramids ---------------------------------------------------
+def resample(src, dst, interp, *, op=capi.RESAMPLE_COPY, src2=None, boundary=capi.CLAMP,
+             const_val=0.0, src_window: Window = None):
+    d = capi.ResampleDesc()
+    d.input = hview(src, src_window)
+    if src2 is not None:
+        d.in2 = hview(src2)
+    d.output = hview(dst)
+    d.interp, d.op, d.boundary, d.const_val = interp, op, boundary, const_val
+    lib().oracle_resample(ctypes.byref(d))
+
+
+BINOMIAL5 = (np.outer([1, 4, 6, 4, 1], [1, 4, 6, 4, 1]) / 256.0).astype(np.float32).ravel()
+
+
+def pyramid(g0, levels, *, mask=BINOMIAL5, boundary=capi.CLAMP, fp_contract=True):
+    """Gaussian and Laplacian pyramids of float image g0 by the pyramid
+    sample's per-level kernel sequence: T = conv5x5(G[l]); G[l+1] = NN(T);
+    L[l] = G[l] - LF(G[l+1]).  Returns (G list, L list of levels-1)."""
+    G, L = [np.ascontiguousarray(g0, np.float32)], []
+    for _ in range(levels - 1):
+        g = G[-1]
+        h, w = g.shape
+        t = np.zeros_like(g)
+        local(g, t, 5, 5, weights=mask, boundary=boundary, fp_contract=fp_contract)
+        g1 = np.zeros((h // 2, w // 2), np.float32)
+        resample(t, g1, capi.INTERP_NN)
+        lap = np.zeros_like(g)
+        resample(g1, lap, capi.INTERP_LF, op=capi.RESAMPLE_SUB, src2=g)
+        G.append(g1)
+        L.append(lap)
+    return G, L

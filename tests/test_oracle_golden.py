@@ -46,6 +46,10 @@ def test_oracle_matches_reference_dsl(case, expected):
         ob.point(capi.POINT_SUB, a, dst, b)
         np.testing.assert_array_equal(dst, expected)
     elif op in ("nn_f32", "lf_f32"):
-        pytest.skip("resampling accessors are pinned by the pyramid tests")
+        dw, dh = int(case["args"][0]), int(case["args"][1])
+        dst = np.zeros((dh, dw), np.float32)
+        interp = capi.INTERP_NN if op == "nn_f32" else capi.INTERP_LF
+        ob.resample(src, dst, interp, boundary=case["boundary"])
+        np.testing.assert_array_equal(dst, expected)
     else:
         pytest.fail(f"unmapped case {op}")
