@@ -7,9 +7,14 @@ operator launch over one 4096x4096 image.  Four input/output buffer pairs
 (512 MiB) are rotated so the working set exceeds the 256 MB Infinity Cache.
 
 The same run also times the rest of the 4096x4096 operator suite (Sobel,
-separable 13x13 Gaussian, 7x7 box sum, reductions, histogram, bilateral) and
-reports them under ``ops``; the CPU baseline is the C oracle (OpenMP
-restatement of the reference's generated CPU backend) on a bounded sample.
+separable 13x13 Gaussian, 7x7 box sum, reductions, histogram, the bilateral
+filter on a smooth field with its error against the FP64 oracle) under
+``ops``; the training step of a learnable 9x9 convolution and 7x7 bilateral
+on 16 x 1024^2 images (plain forward vs forward + backward) under
+``fwd_bwd``; and batched 8-level pyramids of 64 uneven images against
+per-level launches under ``pyramid``.  The CPU baseline is the C oracle
+(OpenMP restatement of the reference's generated CPU backend) on a bounded
+sample.
 
 Before timing, the headline operator's kernel variant is chosen by timing
 each candidate (row-streaming segment heights, wide streaming, LDS tile) on
@@ -95,6 +100,172 @@ def op_entry(name, ms, bytes_per_px, npix=NPIX, extra=None):
     if extra:
         d.update(extra)
     return d
+
+
+def smooth_field(shape, seed, dev):
+    """Smooth random field: sum of 8 seeded sinusoids rescaled to [0, 1], plus
+    N(0, 0.05) noise, clipped to [0, 1] (float32; synthesised with torch)."""
+    import torch
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+    *lead, h, w = shape
+    n = int(np.prod(lead)) if lead else 1
+    yy = torch.arange(h, dtype=torch.float64, device=dev).view(h, 1) / h
+    xx = torch.arange(w, dtype=torch.float64, device=dev).view(1, w) / w
+    out = torch.empty((n, h, w), dtype=torch.float32, device=dev)
+    for i in range(n):
+        p = torch.rand((8, 4), generator=gen, dtype=torch.float64)
+        f = torch.zeros((h, w), dtype=torch.float64, device=dev)
+        for k in range(8):
+            a, u, v, ph = p[k, 0].item(), 1 + 7 * p[k, 1].item(), 1 + 7 * p[k, 2].item(), p[k, 3].item()
+            f += a * torch.sin(2 * np.pi * (u * xx + v * yy) + 2 * np.pi * ph)
+        f = (f - f.min()) / (f.max() - f.min())
+        noise = torch.randn((h, w), generator=gen, dtype=torch.float64).to(dev) * 0.05
+        out[i] = (f + noise).clamp_(0.0, 1.0).float()
+    return out.view(*shape)
+
+
+def bench_config3(op, capi, dev, steps):
+    """4096^2 bilateral 13x13, sigma_d 3, sigma_r 0.1, MIRROR on the smooth
+    field; max |err| against the FP64 oracle on 16 interior rows (the oracle
+    runs on those rows plus the 6-row halo)."""
+    import torch
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_binding as ob
+    x = smooth_field((H, W), 3003, dev)
+    y = torch.empty_like(x)
+    call = op.prepare_bilateral(x, y, 13, 3.0, 0.1, boundary=capi.MIRROR)
+    ms = time_calls([call], steps, 2)
+    torch.cuda.synchronize()
+    y0, rows, r = 2040, 16, 6
+    crop = np.ascontiguousarray(x[y0 - r:y0 + rows + r].cpu().numpy())
+    out64 = np.zeros(crop.shape, np.float64)
+    ob.bilateral(crop, np.zeros_like(crop), 13, 3.0, 0.1, boundary=capi.MIRROR, f64_out=out64)
+    got = y[y0:y0 + rows].cpu().numpy().astype(np.float64)
+    err = float(np.max(np.abs(got - out64[r:r + rows])))
+    flop = NPIX * 169 * 8
+    return op_entry("bilateral13_f32_mirror", ms, 8,
+                    extra={"input": "smooth field (8 sinusoids + N(0,0.05), clipped)",
+                           "gflop_s": round(flop / (ms * 1e-3) / 1e9, 1), "flop_per_px": 169 * 8,
+                           "exp_per_px": 169, "max_abs_err_vs_fp64": err,
+                           "err_rows": [y0, y0 + rows]})
+
+
+def bench_config4(op, capi, dev, steps):
+    """Batch 16 x 1024^2 (smooth field): learnable 9x9 convolution then
+    bilateral 7x7 with learnable sigma_r; plain forward vs forward+backward
+    (training forward, bilateral backward dX + dsigma, convolution backward
+    dX + dw), per boundary mode.  Kernel time of the C-ABI calls."""
+    import torch
+    B, h, w = 16, 1024, 1024
+    x = smooth_field((B, h, w), 4004, dev)
+    g = torch.randn((B, h, w), generator=torch.Generator(device="cpu").manual_seed(4005)).to(dev)
+    wm = (0.1 * np.random.default_rng(4006).standard_normal(81)).astype(np.float32)
+    wd = torch.from_numpy(wm).to(dev)
+    y1, y2, nrm = torch.empty_like(x), torch.empty_like(x), torch.empty_like(x)
+    g1, gx = torch.empty_like(x), torch.empty_like(x)
+    gw = torch.empty(81, dtype=torch.float32, device=dev)
+    gs = torch.empty(1, dtype=torch.float32, device=dev)
+    out = {}
+    for mode, name in ((capi.CLAMP, "clamp"), (capi.MIRROR, "mirror")):
+        conv = op.prepare_local(x, y1, 9, 9, weights_dev=wd, boundary=mode)
+        bil = op.prepare_bilateral(y1, y2, 7, 2.0, 0.1, boundary=mode)
+        bil_t = op.prepare_bilateral_train(y1, y2, nrm, 7, 2.0, 0.1, boundary=mode)
+        bil_b = op.prepare_bilateral_grad(y1, y2, nrm, g, g1, 7, 2.0, 0.1, grad_sigma=gs,
+                                          boundary=mode)
+        conv_b = op.prepare_local_grad(x, g1, gx, 9, 9, weights_dev=wd, grad_weights=gw,
+                                       boundary=mode)
+
+        class Seq:
+            def __init__(self, calls):
+                self.calls = calls
+
+            def launch(self):
+                for c in self.calls:
+                    c.launch()
+
+        fwd = time_calls([Seq([conv, bil])], steps, 3)
+        fb = time_calls([Seq([conv, bil_t, bil_b, conv_b])], steps, 3)
+        parts = {k: round(time_calls([c], steps, 3), 5)
+                 for k, c in (("conv_fwd", conv), ("bilateral_fwd", bil),
+                              ("bilateral_train_fwd", bil_t), ("bilateral_bwd", bil_b),
+                              ("conv_bwd", conv_b))}
+        out[name] = {"fwd_ms": round(fwd, 5), "fwd_bwd_ms": round(fb, 5),
+                     "ratio": round(fb / fwd, 3), "parts_ms": parts,
+                     "conv_ratio": round((parts["conv_fwd"] + parts["conv_bwd"]) /
+                                         parts["conv_fwd"], 3),
+                     "bilateral_ratio": round((parts["bilateral_train_fwd"] +
+                                               parts["bilateral_bwd"]) / parts["bilateral_fwd"], 3)}
+    return {"workload": "16 x 1024 x 1024 f32 smooth field; 9x9 learnable conv -> 7x7 bilateral "
+                        "(sigma_d 2, learnable sigma_r 0.1)",
+            "modes": out, "mpix_per_step": B * h * w / 1e6}
+
+
+def bench_config5(op, capi, dev, reps):
+    """64 float images, W ~ U{256..4096}, H ~ U{256..2160} (seeded), 8-level
+    Gaussian / Laplacian pyramids (5x5 binomial, CLAMP): the batched plan
+    (one persistent launch per level over all images, replayed as a hipGraph)
+    against the pyramid sample's per-image, per-level launches (convolution,
+    NN subsample, LF difference), issued eagerly and also captured in a graph."""
+    import torch
+    rng = np.random.default_rng(5005)
+    n, levels = 64, 8
+    shapes = [(int(rng.integers(256, 2161)), int(rng.integers(256, 4097))) for _ in range(n)]
+    gauss, lap = [], []
+    for i, (h, w) in enumerate(shapes):
+        g0 = torch.empty((h, w), dtype=torch.float32, device=dev)
+        op.fill_random(g0, 6000 + i)
+        gauss.append([g0] + [torch.empty((h >> l, w >> l), dtype=torch.float32, device=dev)
+                             for l in range(1, levels)])
+        lap.append([torch.empty((h >> l, w >> l), dtype=torch.float32, device=dev)
+                    for l in range(levels - 1)])
+    px_levels = sum((h >> l) * (w >> l) for h, w in shapes for l in range(levels - 1))
+    # tile height chosen by timing each candidate (the pyramid's autotuning)
+    best = None
+    for tr in (8, 16, 32):
+        plan = op.PyramidPlan(gauss, lap, tile_rows=tr)
+        ms = time_calls([type("R", (), {"launch": plan.run})()], reps, 2)
+        if best is None or ms < best[0]:
+            best = (ms, tr, plan)
+        else:
+            plan.close()
+    ms_graph, tile_rows, plan = best
+    ms_direct = time_calls([type("R", (), {"launch": plan.run_direct})()], reps, 2)
+    # per-level launches of the sample's kernels
+    g5 = (np.outer([1, 4, 6, 4, 1], [1, 4, 6, 4, 1]) / 256.0).astype(np.float32).ravel()
+    tmp = torch.empty(max(h * w for h, w in shapes), dtype=torch.float32, device=dev)
+    calls = []
+    for i, (h, w) in enumerate(shapes):
+        for l in range(levels - 1):
+            hl, wl = h >> l, w >> l
+            t = tmp[:hl * wl].view(hl, wl)
+            calls.append(op.prepare_local(gauss[i][l], t, 5, 5, weights=g5, boundary=capi.CLAMP))
+            calls.append(op.prepare_resample(t, gauss[i][l + 1], capi.INTERP_NN))
+            calls.append(op.prepare_resample(gauss[i][l + 1], lap[i][l], capi.INTERP_LF,
+                                             op=capi.RESAMPLE_SUB, src2=gauss[i][l]))
+
+    class Seq:
+        def launch(self):
+            for c in calls:
+                c.launch()
+
+    ms_eager = time_calls([Seq()], max(3, reps // 4), 1)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        Seq().launch()
+    ms_eager_graph = time_calls([type("R", (), {"launch": graph.replay})()], reps, 2)
+    plan.close()
+    gbs = 9 * px_levels / (ms_graph * 1e-3) / 1e9
+    return {"workload": "64 f32 images W~U{256..4096} H~U{256..2160}, 8 levels, 5x5 binomial, CLAMP",
+            "tile_rows": tile_rows, "batched_graph_ms": round(ms_graph, 4),
+            "batched_direct_ms": round(ms_direct, 4),
+            "per_level_eager_ms": round(ms_eager, 4),
+            "per_level_graph_ms": round(ms_eager_graph, 4),
+            "speedup_vs_per_level_eager": round(ms_eager / ms_graph, 2),
+            "speedup_vs_per_level_graph": round(ms_eager_graph / ms_graph, 2),
+            "launches_per_level_path": len(calls), "launches_batched": levels - 1,
+            "level_mpix": round(px_levels / 1e6, 1),
+            "mpix_s": round(px_levels / (ms_graph * 1e-3) / 1e6, 1),
+            "gbs_at_9B_per_level_px": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
 def cpu_baseline(budget_s: float):
@@ -200,7 +371,8 @@ def main() -> int:
         try:
             t = json.loads(traffic.read_text()).get("conv5x5_f32_clamp")
             if t:
-                result["roofline"]["traffic"] = t
+                result["roofline"]["traffic"] = t["bytes"]
+                result["roofline"]["traffic_detail"] = t
         except ValueError:
             pass
 
@@ -260,14 +432,9 @@ def main() -> int:
                                   for i in range(NSETS)], 1)
         bins = torch.zeros((NSETS, 256), dtype=torch.int32, device=dev)
         run("histogram256_u8", [op.HistogramCall(u8_in[i], 256, bins[i]) for i in range(NSETS)], 1)
-        bil = [op.prepare_bilateral(f32_in[i], f32_out[i], 13, 3.0, 0.1, boundary=capi.MIRROR)
-               for i in range(NSETS)]
-        bms = time_calls(bil, 12, 2)
-        flop = NPIX * 169 * 8
-        run_entry = op_entry("bilateral13_f32_mirror", bms, 8,
-                             extra={"gflop_s": round(flop / (bms * 1e-3) / 1e9, 1),
-                                    "flop_per_px": 169 * 8, "exp_per_px": 169})
-        ops["bilateral13_f32_mirror"] = run_entry
+        ops["bilateral13_f32_mirror"] = bench_config3(op, capi, dev, 12)
+        result["fwd_bwd"] = bench_config4(op, capi, dev, 20)
+        result["pyramid"] = bench_config5(op, capi, dev, 20)
         del u8_in, u8_out, i16_out
     result["ops"] = ops
     result["cpu_baseline"] = cpu_baseline(args.cpu_seconds if not args.quick else 2.0)

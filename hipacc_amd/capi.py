@@ -124,7 +124,8 @@ SIGNATURES = {
     "hipacc_hip_reduce_sync": (c_int32, [POINTER(View), c_int32, c_int32, c_void_p, c_void_p]),
     "hipacc_hip_histogram": (c_int32, [POINTER(View), c_int32, c_float, c_void_p, c_void_p]),
     "hipacc_hip_bilateral": (c_int32, [POINTER(BilateralDesc), c_void_p]),
-    "hipacc_hip_grad_workspace_bytes": (c_size_t, []),
+    "hipacc_hip_local_grad_workspace_bytes": (c_size_t, [POINTER(LocalGradDesc)]),
+    "hipacc_hip_bilateral_grad_workspace_bytes": (c_size_t, [POINTER(BilateralGradDesc)]),
     "hipacc_hip_resample": (c_int32, [POINTER(ResampleDesc), c_void_p]),
     "hipacc_hip_pyramid_plan_create": (c_int32, [POINTER(PyramidDesc), c_void_p,
                                                  POINTER(c_void_p)]),

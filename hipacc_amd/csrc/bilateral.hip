@@ -1,7 +1,7 @@
 // bilateral.hip -- the Bilateral_Filter sample's operator
 // (samples-public/3_Preprocessing/Bilateral_Filter/src/main.cpp:63-77).
 //
-// Same tile staging and row streaming as the local SUM kernels; per tap the
+// Tile staging as the local SUM kernels; per tap the
 // range weight and the spatial weight are combined in the log2 domain,
 //     s = exp2(-c_r*log2(e) * diff^2 + log2(spatial)),
 // so a tap costs one subtract, one multiply, one FMA, one v_exp_f32 and two
@@ -11,16 +11,19 @@
 
 namespace hh {
 
-template <typename TI, typename TO, int KS, int PPT>
+// 64 x 16 output tiles; each lane owns 4 adjacent pixels of one row.  The
+// input tile with its halo is staged in LDS as float; the tap rows are
+// walked one at a time (not unrolled), so a lane holds one row window of
+// 4 + 2R values plus its 8 sums, whatever the mask size.
+template <typename TI, typename TO, int KS>
 __global__ __launch_bounds__(256) void bilateral_kernel(const BilatArgs a) {
-  constexpr int VEC = 4;  // 4 pixels per lane for both float and uchar images
+  constexpr int VEC = 4;
   constexpr int R = KS / 2;
-  constexpr int HV = (R + VEC - 1) / VEC;
-  constexpr int NV = 64 + 2 * HV;
-  constexpr int LW = NV * VEC;
-  constexpr int TW = 64 * VEC, TH = 4 * PPT;
-  constexpr int LH = TH + 2 * R;
-  constexpr int NW = 1 + 2 * HV;
+  constexpr int HV = (R + VEC - 1) / VEC;   // halo in 4-pixel groups
+  constexpr int TW = 64, TH = 16;
+  constexpr int NV = TW / VEC + 2 * HV;
+  constexpr int LW = NV * VEC, LH = TH + 2 * R;
+  constexpr int M = VEC + 2 * R;
   __shared__ __align__(16) float tile[LH * LW];
 
   const Geom& g = a.g;
@@ -55,71 +58,53 @@ __global__ __launch_bounds__(256) void bilateral_kernel(const BilatArgs a) {
   }
   __syncthreads();
 
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const float* base = tile + (wave * PPT) * LW + lane * VEC;
-  float ctr[PPT][VEC], num[PPT][VEC], den[PPT][VEC];
+  const int tr = threadIdx.x >> 4, tc = (threadIdx.x & 15) * VEC;
+  float ctr[VEC], num[VEC], den[VEC];
 #pragma unroll
-  for (int o = 0; o < PPT; ++o) {
-    const float* c = base + (o + R) * LW + HV * VEC;
-#pragma unroll
-    for (int v = 0; v < VEC; ++v) {
-      ctr[o][v] = c[v];
-      num[o][v] = 0.0f;
-      den[o][v] = 0.0f;
-    }
+  for (int v = 0; v < VEC; ++v) {
+    ctr[v] = tile[(tr + R) * LW + HV * VEC + tc + v];
+    num[v] = 0.0f;
+    den[v] = 0.0f;
   }
   const float k2 = a.k2;
+#pragma unroll 1
+  for (int dy = 0; dy < KS; ++dy) {
+    float win[M];
+    const float* row = tile + (tr + dy) * LW + HV * VEC - R + tc;
 #pragma unroll
-  for (int ir = 0; ir < PPT + 2 * R; ++ir) {
-    float win[NW * VEC];
-    load_window<float, float, VEC, NW>(win, base + ir * LW);
+    for (int m = 0; m < M; ++m) win[m] = row[m];
 #pragma unroll
-    for (int o = 0; o < PPT; ++o) {
-      const int dy = ir - o;
-      if (dy < 0 || dy >= KS) continue;
+    for (int dx = 0; dx < KS; ++dx) {
+      const float l = a.lsp[dy * KS + dx];
 #pragma unroll
-      for (int dx = 0; dx < KS; ++dx) {
-        const float l = a.lsp[dy * KS + dx];
-#pragma unroll
-        for (int v = 0; v < VEC; ++v) {
-          const float x = win[HV * VEC + v + dx - R];
-          const float d = x - ctr[o][v];
-          const float s = __builtin_amdgcn_exp2f(__builtin_fmaf(k2, d * d, l));
-          den[o][v] += s;
-          num[o][v] = __builtin_fmaf(s, x, num[o][v]);
-        }
+      for (int v = 0; v < VEC; ++v) {
+        const float x = win[v + dx];
+        const float d = x - ctr[v];
+        const float sw = __builtin_amdgcn_exp2f(__builtin_fmaf(k2, d * d, l));
+        den[v] += sw;
+        num[v] = __builtin_fmaf(sw, x, num[v]);
       }
     }
   }
-  const int ox = tx0 + lane * VEC;
+  float r[VEC];
 #pragma unroll
-  for (int o = 0; o < PPT; ++o) {
-    float r[VEC];
-#pragma unroll
-    for (int v = 0; v < VEC; ++v) r[v] = num[o][v] / den[o][v];
-    store_row<TO, float, VEC>(out, g, a.out_vec_ok, ox, ty0 + wave * PPT + o, r);
-  }
+  for (int v = 0; v < VEC; ++v) r[v] = num[v] / den[v];
+  const int ox = tx0 + tc, oy = ty0 + tr;
+  store_row<TO, float, VEC>(out, g, a.out_vec_ok, ox, oy, r);
   if constexpr (std::is_same<TO, float>::value) {
-    if (a.norm) {
-      float* nrm = a.norm + (long long)blockIdx.z * a.norm_plane;
+    if (a.norm && oy < g.is_h) {
+      float* nrm = a.norm + (long long)blockIdx.z * a.norm_plane + (long long)oy * a.norm_stride;
 #pragma unroll
-      for (int o = 0; o < PPT; ++o) {
-        const int y = ty0 + wave * PPT + o;
-        if (y >= g.is_h) continue;
-#pragma unroll
-        for (int v = 0; v < VEC; ++v)
-          if (ox + v < g.is_w) nrm[(long long)y * a.norm_stride + ox + v] = den[o][v];
-      }
+      for (int v = 0; v < VEC; ++v)
+        if (ox + v < g.is_w) nrm[ox + v] = den[v];
     }
   }
 }
 
 template <typename TI, typename TO, int KS>
 static int launch_bilat_t(const BilatArgs& a, hipStream_t s) {
-  constexpr int PPT = KS >= 9 ? 2 : 4;
-  dim3 grid(ceil_div(a.g.is_w, 256), ceil_div(a.g.is_h, 4 * PPT), a.g.batch);
-  hipLaunchKernelGGL((bilateral_kernel<TI, TO, KS, PPT>), grid, dim3(256), 0, s, a);
+  dim3 grid(ceil_div(a.g.is_w, 64), ceil_div(a.g.is_h, 16), a.g.batch);
+  hipLaunchKernelGGL((bilateral_kernel<TI, TO, KS>), grid, dim3(256), 0, s, a);
   return hip_status(hipGetLastError(), "bilateral_kernel launch");
 }
 

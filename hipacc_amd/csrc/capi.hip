@@ -44,13 +44,12 @@ int hip_status(hipError_t e, const char* what) {
 constexpr int kMaxDevices = 64;
 constexpr size_t kRedWorkspace = 1024 * sizeof(unsigned long long) + 256;
 
-constexpr size_t kGradWorkspace = (size_t)kGradBlocks * kMaxTaps * sizeof(float);
-
 struct DeviceState {
   std::mutex mu;
   bool ready = false;
   void* red_ws = nullptr;
-  void* grad_ws = nullptr;   // adjoint partial sums, allocated on first use
+  void* grad_ws = nullptr;   // adjoint workspace, allocated / grown on demand
+  size_t grad_bytes = 0;
 };
 static DeviceState g_dev[kMaxDevices];
 
@@ -76,15 +75,25 @@ static int ensure_workspace(int dev, void** ws) {
   return HIPACC_OK;
 }
 
-static int ensure_grad_workspace(void** ws) {
+// The adjoint workspace of the current device, at least `bytes` long.
+// Growing it frees the old buffer after the device is idle.
+static int ensure_grad_workspace(size_t bytes, void** ws) {
   int dev = 0;
   int st = current_device(&dev);
   if (st) return st;
   DeviceState& d = g_dev[dev];
   std::lock_guard<std::mutex> lk(d.mu);
-  if (!d.grad_ws) {
-    hipError_t e = hipMalloc(&d.grad_ws, kGradWorkspace);
+  if (d.grad_bytes < bytes) {
+    if (d.grad_ws) {
+      hipError_t e = hipDeviceSynchronize();
+      if (e != hipSuccess) return hip_status(e, "hipDeviceSynchronize");
+      (void)hipFree(d.grad_ws);
+      d.grad_ws = nullptr;
+      d.grad_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&d.grad_ws, bytes);
     if (e != hipSuccess) return hip_status(e, "hipMalloc(gradient workspace)");
+    d.grad_bytes = bytes;
   }
   *ws = d.grad_ws;
   return HIPACC_OK;
@@ -720,17 +729,41 @@ int hipacc_hip_bilateral(const hipacc_bilateral_desc* d, void* stream) {
   return tm.finish(launch_bilateral(a, s));
 }
 
-size_t hipacc_hip_grad_workspace_bytes(void) { return kGradWorkspace; }
+}  // extern "C"
 
-int hipacc_hip_local_grad(const hipacc_local_grad_desc* d, void* stream) {
-  if (!d) return set_error(HIPACC_ERR_INVALID_ARG, "local_grad: desc is NULL");
+namespace hh {
+
+static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Workspace layout of the convolution adjoint: [flipped mask][ring][dw partials].
+struct ConvGradWs {
+  size_t flip, ring, part, total;
+};
+static ConvGradWs conv_grad_ws(const hipacc_local_grad_desc* d) {
+  ConvGradWs w;
+  const int kk = d->mask_w * d->mask_h;
   const bool dx = d->grad_in.data != nullptr, dw = d->grad_weights_dev != nullptr;
+  const int nb = d->in.batch > 1 ? d->in.batch : 1;
+  w.flip = 0;
+  w.ring = align256(sizeof(float) * kMaxTaps);
+  const size_t ring = (dx && d->boundary != HIPACC_BOUNDARY_CONSTANT)
+                          ? sizeof(float) * (size_t)nb *
+                                (size_t)ring_size(d->in.width, d->in.height, d->mask_w / 2,
+                                                  d->mask_h / 2)
+                          : 0;
+  w.part = w.ring + align256(ring);
+  w.total = w.part + (dw ? align256(sizeof(float) * (size_t)kGradBlocks * kk) : 0);
+  return w;
+}
+
+static int check_local_grad(const hipacc_local_grad_desc* d) {
+  if (!d) return set_error(HIPACC_ERR_INVALID_ARG, "local_grad: desc is NULL");
   int st = grad_image(d->in, "in");
   if (st) return st;
   if ((st = grad_image(d->grad_out, "grad_out"))) return st;
   if (!same_shape(d->in, d->grad_out))
     return set_error(HIPACC_ERR_INVALID_ARG, "local_grad: grad_out differs from in in size");
-  if (dx) {
+  if (d->grad_in.data) {
     if ((st = grad_image(d->grad_in, "grad_in"))) return st;
     if (!same_shape(d->in, d->grad_in))
       return set_error(HIPACC_ERR_INVALID_ARG, "local_grad: grad_in differs from in in size");
@@ -743,6 +776,31 @@ int hipacc_hip_local_grad(const hipacc_local_grad_desc* d, void* stream) {
     return set_error(HIPACC_ERR_INVALID_ARG, "local_grad: no weights");
   if (d->boundary < HIPACC_BOUNDARY_UNDEFINED || d->boundary > HIPACC_BOUNDARY_CONSTANT)
     return set_error(HIPACC_ERR_INVALID_ARG, "local_grad: boundary %d", d->boundary);
+  return HIPACC_OK;
+}
+
+}  // namespace hh
+
+extern "C" {
+
+size_t hipacc_hip_local_grad_workspace_bytes(const hipacc_local_grad_desc* d) {
+  if (check_local_grad(d)) return 0;
+  return conv_grad_ws(d).total;
+}
+
+int hipacc_hip_local_grad(const hipacc_local_grad_desc* d, void* stream) {
+  int st = check_local_grad(d);
+  if (st) return st;
+  const bool dx = d->grad_in.data != nullptr, dw = d->grad_weights_dev != nullptr;
+  if (!dx && !dw) return HIPACC_OK;
+  const int kx = d->mask_w, ky = d->mask_h, kk = kx * ky;
+  const ConvGradWs lay = conv_grad_ws(d);
+  char* ws = static_cast<char*>(d->workspace);
+  if (!ws) {
+    void* p = nullptr;
+    if ((st = ensure_grad_workspace(lay.total, &p))) return st;
+    ws = static_cast<char*>(p);
+  }
   ConvGradArgs a;
   std::memset(&a, 0, sizeof(a));
   a.x = static_cast<const float*>(d->in.data);
@@ -757,20 +815,57 @@ int hipacc_hip_local_grad(const hipacc_local_grad_desc* d, void* stream) {
   a.w = d->in.width;
   a.h = d->in.height;
   a.batch = batch_of(d->in);
-  a.kx = d->mask_w;
-  a.ky = d->mask_h;
+  a.kx = kx;
+  a.ky = ky;
   a.boundary = d->boundary;
   a.cval = (float)d->const_val;
-  a.vec_ok = rows_aligned(d->in, 16) && rows_aligned(d->grad_out, 16) &&
-             (!dx || rows_aligned(d->grad_in, 16));
+  a.vec_ok = rows_aligned(d->in, 16) && rows_aligned(d->grad_out, 16);
   a.wdev = d->weights_dev;
-  if (!a.wdev) std::memcpy(a.wh, d->weights_f32, sizeof(float) * a.kx * a.ky);
-  void* ws = d->workspace;
-  if (dw && !ws && (st = ensure_grad_workspace(&ws))) return st;
-  a.partials = static_cast<float*>(ws);
+  if (!a.wdev) std::memcpy(a.wh, d->weights_f32, sizeof(float) * kk);
+  a.partials = reinterpret_cast<float*>(ws + lay.part);
   hipStream_t s = static_cast<hipStream_t>(stream);
   Timer tm(s);
-  return tm.finish(launch_conv_grad(a, d->grad_weights_dev, s));
+  if (dx) {
+    // E over the image: the forward operator on dY with the flipped mask and
+    // a zero border.
+    hipacc_local_desc f;
+    std::memset(&f, 0, sizeof(f));
+    f.in = d->grad_out;
+    f.out = d->grad_in;
+    f.mask_w = kx;
+    f.mask_h = ky;
+    float flipped[kMaxTaps];
+    if (d->weights_dev) {
+      float* wf = reinterpret_cast<float*>(ws + lay.flip);
+      if ((st = launch_flip(d->weights_dev, wf, kk, s))) return tm.finish(st);
+      f.weights_dev = wf;
+    } else {
+      for (int t = 0; t < kk; ++t) flipped[t] = d->weights_f32[kk - 1 - t];
+      f.weights_f32 = flipped;
+    }
+    f.use_weights = 1;
+    f.acc_dtype = HIPACC_F32;
+    f.reduce = HIPACC_REDUCE_SUM;
+    f.boundary = HIPACC_BOUNDARY_CONSTANT;
+    f.const_val = 0.0;
+    f.cvt = HIPACC_CVT_TRUNC;
+    f.fp_contract = 1;
+    LocalArgs la;
+    SepArgs sa;
+    int accf = 0;
+    bool box = false;
+    if ((st = prep_local(&f, la, accf, box, sa))) return tm.finish(st);
+    if ((st = launch_local(la, accf, s, kVarAuto, 0))) return tm.finish(st);
+    if (d->boundary != HIPACC_BOUNDARY_CONSTANT) {
+      float* ring = reinterpret_cast<float*>(ws + lay.ring);
+      if ((st = launch_conv_ring(a, ring, s))) return tm.finish(st);
+      if ((st = launch_fold_add(a.gx, a.gx_plane, a.gx_stride, a.w, a.h, a.batch, kx / 2, ky / 2,
+                                d->boundary, ring, s)))
+        return tm.finish(st);
+    }
+  }
+  if (dw) st = launch_conv_dw(a, d->grad_weights_dev, s);
+  return tm.finish(st);
 }
 
 // Spatial table and argument checks shared by the bilateral entry points.
@@ -824,6 +919,17 @@ int hipacc_hip_bilateral_train(const hipacc_bilateral_grad_desc* d, void* stream
   return tm.finish(launch_bilateral(a, s));
 }
 
+size_t hipacc_hip_bilateral_grad_workspace_bytes(const hipacc_bilateral_grad_desc* d) {
+  if (!d) return 0;
+  const int nb = d->in.batch > 1 ? d->in.batch : 1;
+  const int r = d->ksize / 2;
+  const size_t ring = (d->grad_in.data && d->boundary != HIPACC_BOUNDARY_CONSTANT)
+                          ? sizeof(float) * (size_t)nb *
+                                (size_t)ring_size(d->in.width, d->in.height, r, r)
+                          : 0;
+  return align256(ring) + align256(sizeof(float) * kGradBlocks);
+}
+
 int hipacc_hip_bilateral_grad(const hipacc_bilateral_grad_desc* d, void* stream) {
   if (!d) return set_error(HIPACC_ERR_INVALID_ARG, "bilateral_grad: desc is NULL");
   const bool dx = d->grad_in.data != nullptr, ds = d->grad_sigma_r_dev != nullptr;
@@ -838,6 +944,7 @@ int hipacc_hip_bilateral_grad(const hipacc_bilateral_grad_desc* d, void* stream)
   BilatGradArgs a;
   std::memset(&a, 0, sizeof(a));
   if ((st = bilat_common(d, a.sp))) return st;
+  if (!dx && !ds) return HIPACC_OK;
   a.x = static_cast<const float*>(d->in.data);
   a.y = static_cast<const float*>(d->out.data);
   a.d = static_cast<const float*>(d->norm.data);
@@ -863,12 +970,21 @@ int hipacc_hip_bilateral_grad(const hipacc_bilateral_grad_desc* d, void* stream)
   a.k2 = -a.c * 1.44269504088896340736f;
   a.vec_ok = rows_aligned(d->in, 16) && rows_aligned(d->out, 16) && rows_aligned(d->norm, 16) &&
              rows_aligned(d->grad_out, 16) && (!dx || rows_aligned(d->grad_in, 16));
-  void* ws = d->workspace;
-  if (!ws && (st = ensure_grad_workspace(&ws))) return st;
-  a.partials = static_cast<float*>(ws);
+  const size_t total = hipacc_hip_bilateral_grad_workspace_bytes(d);
+  char* ws = static_cast<char*>(d->workspace);
+  if (!ws) {
+    void* p = nullptr;
+    if ((st = ensure_grad_workspace(total, &p))) return st;
+    ws = static_cast<char*>(p);
+  }
+  const size_t part = total - align256(sizeof(float) * kGradBlocks);
+  a.partials = reinterpret_cast<float*>(ws + part);
+  float* ring = (dx && d->boundary != HIPACC_BOUNDARY_CONSTANT) ? reinterpret_cast<float*>(ws)
+                                                                : nullptr;
   hipStream_t s = static_cast<hipStream_t>(stream);
   Timer tm(s);
-  return tm.finish(launch_bilateral_grad(a, ds ? d->grad_sigma_r_dev : nullptr, d->sigma_r, s));
+  return tm.finish(
+      launch_bilateral_grad(a, ds ? d->grad_sigma_r_dev : nullptr, d->sigma_r, ring, s));
 }
 
 int hipacc_hip_resample(const hipacc_resample_desc* d, void* stream) {

@@ -8,44 +8,47 @@
 //   dL/dc                 -SUM_q a(q) SUM_t s_{q,t} d^2 (v - Y(q))
 // Away from the borders the neighbour p = q + o reads q through the opposite
 // tap -o, and exp(-c d^2) is the same for the pair, so one exponential per
-// (q, o) serves both terms.  Pixels within r of an edge add the neighbour
-// terms of the halo positions the boundary mode folds onto them.
+// (q, o) serves both terms.  Pixels within r of an edge also receive the
+// neighbour terms of the halo positions the boundary mode folds onto them:
+// those are evaluated once per halo (ring) position and added afterwards.
 #include "hh_grad.hpp"
 
 namespace hh {
 
-// Folded neighbour terms of border pixel q: taps of in-image pixels p that
-// land on an extended position i != q with bh(i) == q.  Global reads.
-__device__ float bilat_fold(const BilatGradArgs& a, int b, int qx, int qy, float xq) {
-  int px[2 * (HIPACC_MAX_MASK_DIM / 2) + 1], py[2 * (HIPACC_MAX_MASK_DIM / 2) + 1];
+// Neighbour terms landing on ring position i (outside the image), read by
+// the taps of in-image pixels p = i - o_t; the value read there is X(q),
+// q = bh(i).  One thread per ring position of one image.
+__global__ __launch_bounds__(256) void bilat_ring_kernel(const BilatGradArgs a, float* ring,
+                                                         long long rsize) {
+  const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int b = blockIdx.y;
+  if (j >= rsize) return;
   const int r = a.ksize / 2;
-  const int nx = preimage(a.boundary, qx, a.w, r, px);
-  const int ny = preimage(a.boundary, qy, a.h, r, py);
+  int ix, iy;
+  ring_pos(a.w, a.h, r, r, j, ix, iy);
+  const int qx = bh_map(a.boundary, ix, 0, a.w, a.w), qy = bh_map(a.boundary, iy, 0, a.h, a.h);
   const float* X = a.x + b * a.x_plane;
   const float* Y = a.y + b * a.y_plane;
   const float* D = a.d + b * a.d_plane;
   const float* G = a.gy + b * a.gy_plane;
+  const float xq = X[(long long)qy * a.x_stride + qx];
   float acc = 0.0f;
-  for (int j = 0; j < ny; ++j)
-    for (int i = 0; i < nx; ++i) {
-      if (i == 0 && j == 0) continue;
-      for (int ty = 0; ty < a.ksize; ++ty) {
-        const int y = py[j] - (ty - r);
-        if (y < 0 || y >= a.h) continue;
-        for (int tx = 0; tx < a.ksize; ++tx) {
-          const int x = px[i] - (tx - r);
-          if (x < 0 || x >= a.w) continue;
-          const float xp = X[(long long)y * a.x_stride + x];
-          const float yp = Y[(long long)y * a.y_stride + x];
-          const float ap = G[(long long)y * a.gy_stride + x] / D[(long long)y * a.d_stride + x];
-          const float dd = xp - xq;
-          const float s = __builtin_amdgcn_exp2f(a.k2 * (dd * dd)) * a.sp[ty * a.ksize + tx];
-          const float w = ap * s;
-          acc += w + 2.0f * a.c * (w * dd) * (xq - yp);
-        }
-      }
+  for (int ty = 0; ty < a.ksize; ++ty) {
+    const int y = iy - (ty - r);
+    if (y < 0 || y >= a.h) continue;
+    for (int tx = 0; tx < a.ksize; ++tx) {
+      const int x = ix - (tx - r);
+      if (x < 0 || x >= a.w) continue;
+      const float xp = X[(long long)y * a.x_stride + x];
+      const float yp = Y[(long long)y * a.y_stride + x];
+      const float ap = G[(long long)y * a.gy_stride + x] / D[(long long)y * a.d_stride + x];
+      const float dd = xp - xq;
+      const float sv = __builtin_amdgcn_exp2f(a.k2 * (dd * dd)) * a.sp[ty * a.ksize + tx];
+      const float w = ap * sv;
+      acc += w + 2.0f * a.c * (w * dd) * (xq - yp);
     }
-  return acc;
+  }
+  ring[b * rsize + j] = acc;
 }
 
 // 64 x 16 pixel tiles, 4 adjacent pixels per lane.  LDS holds, with a halo of
@@ -178,15 +181,8 @@ __global__ __launch_bounds__(256) void bilat_grad_kernel(const BilatGradArgs a) 
           *reinterpret_cast<float4*>(GX + x0 + tc) = make_float4(v[0], v[1], v[2], v[3]);
         } else {
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const int qx = x0 + tc + j;
-            if (qx >= a.w) continue;
-            float r = v[j];
-            if (a.boundary != HIPACC_BOUNDARY_CONSTANT &&
-                (qx < R || qx >= a.w - R || qy < R || qy >= a.h - R))
-              r += bilat_fold(a, b, qx, qy, xq[j]);
-            GX[qx] = r;
-          }
+          for (int j = 0; j < 4; ++j)
+            if (x0 + tc + j < a.w) GX[x0 + tc + j] = v[j];
         }
       }
     }
@@ -219,7 +215,8 @@ static int launch_bg(const BilatGradArgs& a, int nblocks, hipStream_t s) {
   return hip_status(hipGetLastError(), "bilat_grad_kernel launch");
 }
 
-int launch_bilateral_grad(const BilatGradArgs& a0, float* dsigma, float sigma_r, hipStream_t s) {
+int launch_bilateral_grad(const BilatGradArgs& a0, float* dsigma, float sigma_r, float* ring,
+                          hipStream_t s) {
   BilatGradArgs a = a0;
   if (!a.gx && !dsigma) return HIPACC_OK;
   a.tiles_x = ceil_div(a.w, 64);
@@ -237,6 +234,18 @@ int launch_bilateral_grad(const BilatGradArgs& a0, float* dsigma, float sigma_r,
     case 13: st = launch_bg<6>(a, nblocks, s); break;
     case 15: st = launch_bg<7>(a, nblocks, s); break;
     default: return set_error(HIPACC_ERR_UNSUPPORTED, "bilateral_grad: ksize %d", a.ksize);
+  }
+  if (st == HIPACC_OK && a.gx && ring && a.boundary != HIPACC_BOUNDARY_CONSTANT) {
+    const int r = a.ksize / 2;
+    const long long rs = ring_size(a.w, a.h, r, r);
+    if (rs > 0) {
+      dim3 grid((unsigned)((rs + 255) / 256), a.batch);
+      hipLaunchKernelGGL(bilat_ring_kernel, grid, dim3(256), 0, s, a, ring, rs);
+      st = hip_status(hipGetLastError(), "bilat_ring_kernel launch");
+      if (st == HIPACC_OK)
+        st = launch_fold_add(a.gx, a.gx_plane, a.gx_stride, a.w, a.h, a.batch, r, r, a.boundary,
+                             ring, s);
+    }
   }
   if (st == HIPACC_OK && dsigma) {
     const double sr = sigma_r;

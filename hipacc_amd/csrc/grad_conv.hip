@@ -3,27 +3,45 @@
 // over whole images with boundary mode bh:
 //     dX(q) = SUM_{p,t : bh(p + t - r) = q} w[t] * dY(p)
 //     dw[t] = SUM_p dY(p) * X_bh(p + t - r)
-// dX is computed in gather form.  With E(i) = SUM_t w[t] * dY0(i - t + r)
-// (dY0 = dY inside the image, 0 outside) over the extended index range
-// i in [-r, n-1+r], and P(q) = { i : bh(i) = q } (separable per axis),
+// dX in gather form: with E(i) = SUM_t w[t] * dY0(i + r - t) (dY0 = dY inside
+// the image, 0 outside) over the extended index range, and P(q) = { i :
+// bh(i) = q } (separable per axis),
 //     dX(q) = SUM_{i in P(q)} E(i).
-// Away from the borders P(q) = {q} and dX is the flipped-mask correlation of
-// dY; pixels within r of an edge add the folded terms of the halo positions
-// that the boundary mode maps onto them (CLAMP: the run of clamped positions,
-// MIRROR: the reflected position, REPEAT: the wrapped one; CONSTANT: none).
-// dw is a per-tap reduction: per-block partial sums, then a fixed-order
-// reduction (deterministic run to run).
+// E over the image is the forward operator itself applied to dY with the
+// flipped mask and CONSTANT 0 borders (the caller runs it through the
+// forward's kernels).  The rest of P(q) -- the halo positions the mode folds
+// onto pixels within r of an edge (CLAMP: the clamped run, MIRROR: the
+// reflection, REPEAT: the wrap; CONSTANT: none) -- is evaluated once per
+// ring position here and added to the border band.  dw: per-block partial
+// sums, then a fixed-order reduction (deterministic run to run).
 #include "hh_grad.hpp"
 
 namespace hh {
 
 __device__ __forceinline__ float grad_weight(const ConvGradArgs& a, int t) {
-  return a.wdev ? a.wdev[t] : a.wh[t];
+  return a.wdev ? ((const HH_CONST float*)a.wdev)[t] : a.wh[t];
 }
 
-// E(ix, iy) read from global memory (any extended position).
-__device__ float conv_e(const ConvGradArgs& a, const float* G, int ix, int iy) {
+__global__ void flip_kernel(const float* w, float* out, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = w[n - 1 - t];
+}
+
+int launch_flip(const float* w, float* out, int n, hipStream_t s) {
+  hipLaunchKernelGGL(flip_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, w, out, n);
+  return hip_status(hipGetLastError(), "flip_kernel launch");
+}
+
+// One thread per ring position of one image.
+__global__ __launch_bounds__(256) void conv_ring_kernel(const ConvGradArgs a, float* ring,
+                                                        long long rsize) {
+  const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int b = blockIdx.y;
+  if (j >= rsize) return;
   const int rx = a.kx / 2, ry = a.ky / 2;
+  int ix, iy;
+  ring_pos(a.w, a.h, rx, ry, j, ix, iy);
+  const float* G = a.gy + b * a.gy_plane;
   float e = 0.0f;
   for (int ty = 0; ty < a.ky; ++ty) {
     const int y = iy + ry - ty;
@@ -34,162 +52,212 @@ __device__ float conv_e(const ConvGradArgs& a, const float* G, int ix, int iy) {
       e = __builtin_fmaf(grad_weight(a, ty * a.kx + tx), G[(long long)y * a.gy_stride + x], e);
     }
   }
-  return e;
+  ring[b * rsize + j] = e;
 }
 
-// SUM of E over P(qx) x P(qy), without (qx, qy) itself when skip_self.
-__device__ float conv_fold(const ConvGradArgs& a, const float* G, int qx, int qy, bool skip_self) {
+int launch_conv_ring(const ConvGradArgs& a, float* ring, hipStream_t s) {
+  const long long rs = ring_size(a.w, a.h, a.kx / 2, a.ky / 2);
+  if (rs == 0) return HIPACC_OK;
+  dim3 grid((unsigned)((rs + 255) / 256), a.batch);
+  hipLaunchKernelGGL(conv_ring_kernel, grid, dim3(256), 0, s, a, ring, rs);
+  return hip_status(hipGetLastError(), "conv_ring_kernel launch");
+}
+
+// One thread per band pixel of one image.
+__global__ __launch_bounds__(256) void fold_add_kernel(float* gx, long long gx_plane,
+                                                       int gx_stride, int w, int h, int rx,
+                                                       int ry, int mode, const float* ring,
+                                                       long long rsize, long long bsize) {
+  const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int b = blockIdx.y;
+  if (j >= bsize) return;
+  int qx, qy;
+  band_pos(w, h, rx, ry, j, qx, qy);
   int px[2 * (HIPACC_MAX_MASK_DIM / 2) + 1], py[2 * (HIPACC_MAX_MASK_DIM / 2) + 1];
-  const int nx = preimage(a.boundary, qx, a.w, a.kx / 2, px);
-  const int ny = preimage(a.boundary, qy, a.h, a.ky / 2, py);
+  const int nx = preimage(mode, qx, w, rx, px);
+  const int ny = preimage(mode, qy, h, ry, py);
+  if (nx == 1 && ny == 1) return;
+  const float* R = ring + b * rsize;
   float acc = 0.0f;
-  for (int j = 0; j < ny; ++j)
-    for (int i = 0; i < nx; ++i) {
-      if (skip_self && i == 0 && j == 0) continue;
-      acc += conv_e(a, G, px[i], py[j]);
+  for (int jy = 0; jy < ny; ++jy)
+    for (int jx = 0; jx < nx; ++jx) {
+      if (jx == 0 && jy == 0) continue;
+      acc += R[ring_index(w, h, rx, ry, px[jx], py[jy])];
     }
-  return acc;
+  float* dst = gx + b * gx_plane + (long long)qy * gx_stride + qx;
+  *dst = *dst + acc;
 }
 
-// dX and/or dw over 64 x 16 pixel tiles of square K = 2R+1 masks.  Each
-// lane owns 4 adjacent pixels of one row; dY and X tiles with halo are
-// staged in LDS; a block walks tiles blockIdx.x, +gridDim.x, ... and keeps
-// its dw partial sums (K*K per lane) in registers.
-template <int R, bool DX, bool DW>
-__global__ __launch_bounds__(256) void conv_grad_kernel(const ConvGradArgs a) {
+int launch_fold_add(float* gx, long long gx_plane, int gx_stride, int w, int h, int batch, int rx,
+                    int ry, int mode, const float* ring, hipStream_t s) {
+  if (mode == HIPACC_BOUNDARY_CONSTANT) return HIPACC_OK;
+  const long long bs = band_size(w, h, rx, ry), rs = ring_size(w, h, rx, ry);
+  if (bs == 0) return HIPACC_OK;
+  dim3 grid((unsigned)((bs + 255) / 256), batch);
+  hipLaunchKernelGGL(fold_add_kernel, grid, dim3(256), 0, s, gx, gx_plane, gx_stride, w, h, rx,
+                     ry, mode, ring, rs, bs);
+  return hip_status(hipGetLastError(), "fold_add_kernel launch");
+}
+
+// dw over 64 x 32 pixel tiles of square K = 2R+1 masks.  Each lane owns 4
+// adjacent pixels in each of 2 consecutive rows (the two rows share K-1 of
+// their K+1 X rows) and keeps K*K partial sums; the X tile with its halo and
+// the dY tile are staged in LDS.  A block walks a contiguous range of tiles
+// and loads the next interior tile into registers before computing the
+// current one.
+// Part i of a tile (16-byte group i of LH x LW4 groups) from global memory,
+// and into LDS.
+template <int LW4, int NX4, int R, int RH>
+__device__ __forceinline__ void tile_part_load(float4& p, int i, const float* X, int stride,
+                                               int x0, int y0) {
+  if (i < NX4) {
+    const int r = i / LW4, c = (i - r * LW4) * 4;
+    p = *reinterpret_cast<const float4*>(X + (long long)(y0 - R + r) * stride + (x0 - RH + c));
+  }
+}
+template <int LW4, int NX4>
+__device__ __forceinline__ void tile_part_store(const float4& p, int i, float* xs) {
+  if (i < NX4) {
+    const int r = i / LW4, c = (i - r * LW4) * 4;
+    *reinterpret_cast<float4*>(xs + r * (LW4 * 4) + c) = p;
+  }
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void conv_dw_kernel(const ConvGradArgs a, int per_block) {
   constexpr int K = 2 * R + 1, KK = K * K;
-  constexpr int TW = 64, TH = 16;
+  constexpr int TW = 64, TH = 32;
   constexpr int RH = (R + 3) & ~3;   // column halo rounded to 16-byte groups
   constexpr int LW = TW + 2 * RH, LH = TH + 2 * R, LW4 = LW / 4;
+  constexpr int NX4 = LH * LW4, NL = (NX4 + 255) / 256;
+  static_assert(NL <= 3, "tile staging assumes at most 3 parts per thread");
   constexpr int M = 4 + 2 * R;
-  __shared__ __align__(16) float gs[LH * LW];
-  __shared__ __align__(16) float xs[DW ? LH * LW : 4];
-  __shared__ float red[DW ? 4 * KK : 4];
+  __shared__ __align__(16) float xs[LH * LW];
+  __shared__ __align__(16) float gs[TH * TW];
+  __shared__ float red[4 * KK];
 
-  const int tid = threadIdx.x, tr = tid >> 4, tc = (tid & 15) * 4;
-  float dw[DW ? KK : 1];
+  const int tid = threadIdx.x, tr = (tid >> 4) * 2, tc = (tid & 15) * 4;
+  float dw[KK];
 #pragma unroll
-  for (int t = 0; t < (DW ? KK : 1); ++t) dw[t] = 0.0f;
-  const int per_plane = a.tiles_x * a.tiles_y;
+  for (int t = 0; t < KK; ++t) dw[t] = 0.0f;
 
-  for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
-    const int b = tile / per_plane, rem = tile - b * per_plane;
-    const int y0 = (rem / a.tiles_x) * TH, x0 = (rem % a.tiles_x) * TW;
-    const float* X = a.x + b * a.x_plane;
-    const float* G = a.gy + b * a.gy_plane;
-    const bool interior =
-        a.vec_ok && x0 >= RH && x0 + TW + RH <= a.w && y0 >= R && y0 + TH + R <= a.h;
+  const int t0 = blockIdx.x * per_block;
+  const int t1 = min(t0 + per_block, a.ntiles);
+  // tile coordinates, advanced incrementally
+  int b = 0, ty = 0, tx = 0;
+  if (t0 < t1) {
+    const int per_plane = a.tiles_x * a.tiles_y;
+    b = t0 / per_plane;
+    const int rem = t0 - b * per_plane;
+    ty = rem / a.tiles_x;
+    tx = rem - ty * a.tiles_x;
+  }
+  auto interior = [&](int x0, int y0) {
+    return a.vec_ok && x0 >= RH && x0 + TW + RH <= a.w && y0 >= R && y0 + TH + R <= a.h;
+  };
+  float4 p0, p1, p2, pg0, pg1;
+  auto prefetch = [&](int pb, int x0, int y0) {
+    const float* X = a.x + pb * a.x_plane;
+    const float* G = a.gy + pb * a.gy_plane;
+    tile_part_load<LW4, NX4, R, RH>(p0, tid, X, a.x_stride, x0, y0);
+    if constexpr (NL > 1) tile_part_load<LW4, NX4, R, RH>(p1, tid + 256, X, a.x_stride, x0, y0);
+    if constexpr (NL > 2) tile_part_load<LW4, NX4, R, RH>(p2, tid + 512, X, a.x_stride, x0, y0);
+    pg0 = *reinterpret_cast<const float4*>(G + (long long)(y0 + tr) * a.gy_stride + x0 + tc);
+    pg1 = *reinterpret_cast<const float4*>(G + (long long)(y0 + tr + 1) * a.gy_stride + x0 + tc);
+  };
+
+  bool cur = t0 < t1 && interior(tx * TW, ty * TH);
+  if (cur) prefetch(b, tx * TW, ty * TH);
+  for (int t = t0; t < t1; ++t) {
+    const int x0 = tx * TW, y0 = ty * TH;
     __syncthreads();  // the previous tile's LDS reads are done
-    if (interior) {
-      constexpr int N4 = LH * LW4;
-      for (int i = tid; i < N4; i += 256) {
-        const int r = i / LW4, c = (i - r * LW4) * 4;
-        const long long col = x0 - RH + c;
-        const int row = y0 - R + r;
-        const float4 gv = *reinterpret_cast<const float4*>(G + (long long)row * a.gy_stride + col);
-        if constexpr (DW) {
-          const float4 xv = *reinterpret_cast<const float4*>(X + (long long)row * a.x_stride + col);
-          *reinterpret_cast<float4*>(xs + r * LW + c) = xv;
-        }
-        *reinterpret_cast<float4*>(gs + r * LW + c) = gv;
-      }
+    if (cur) {
+      tile_part_store<LW4, NX4>(p0, tid, xs);
+      if constexpr (NL > 1) tile_part_store<LW4, NX4>(p1, tid + 256, xs);
+      if constexpr (NL > 2) tile_part_store<LW4, NX4>(p2, tid + 512, xs);
+      *reinterpret_cast<float4*>(gs + tr * TW + tc) = pg0;
+      *reinterpret_cast<float4*>(gs + (tr + 1) * TW + tc) = pg1;
     } else {
+      const float* X = a.x + b * a.x_plane;
+      const float* G = a.gy + b * a.gy_plane;
       for (int i = tid; i < LH * LW; i += 256) {
         const int r = i / LW, c = i - r * LW;
         const int y = y0 - R + r, x = x0 - RH + c;
-        const bool inside = x >= 0 && x < a.w && y >= 0 && y < a.h;
-        gs[i] = inside ? G[(long long)y * a.gy_stride + x] : 0.0f;
-        if constexpr (DW) {
-          float v;
-          if (inside)
-            v = X[(long long)y * a.x_stride + x];
-          else if (a.boundary == HIPACC_BOUNDARY_CONSTANT)
-            v = a.cval;
-          else
-            v = X[(long long)bh_map(a.boundary, y, 0, a.h, a.h) * a.x_stride +
-                  bh_map(a.boundary, x, 0, a.w, a.w)];
-          xs[i] = v;
-        }
+        float v;
+        if (x >= 0 && x < a.w && y >= 0 && y < a.h)
+          v = X[(long long)y * a.x_stride + x];
+        else if (a.boundary == HIPACC_BOUNDARY_CONSTANT)
+          v = a.cval;
+        else
+          v = X[(long long)bh_map(a.boundary, y, 0, a.h, a.h) * a.x_stride +
+                bh_map(a.boundary, x, 0, a.w, a.w)];
+        xs[i] = v;
       }
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int y = y0 + tr + q, x = x0 + tc + j;
+          gs[(tr + q) * TW + tc + j] =
+              (x < a.w && y < a.h) ? G[(long long)y * a.gy_stride + x] : 0.0f;
+        }
     }
     __syncthreads();
-
-    float gx[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    float gc[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) gc[j] = gs[(tr + R) * LW + tc + RH + j];
-#pragma unroll
-    for (int dy = 0; dy < K; ++dy) {
-      float gr[M], xr[DW ? M : 1];
-      const int off = (tr + dy) * LW + tc + (RH - R);
-#pragma unroll
-      for (int m = 0; m < M; ++m) {
-        gr[m] = gs[off + m];
-        if constexpr (DW) xr[DW ? m : 0] = xs[off + m];
+    // advance to the next tile and prefetch it
+    if (++tx == a.tiles_x) {
+      tx = 0;
+      if (++ty == a.tiles_y) {
+        ty = 0;
+        ++b;
       }
+    }
+    const bool nxt = t + 1 < t1 && interior(tx * TW, ty * TH);
+    if (nxt) prefetch(b, tx * TW, ty * TH);
+
+    float g0[4], g1[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      g0[j] = gs[tr * TW + tc + j];
+      g1[j] = gs[(tr + 1) * TW + tc + j];
+    }
+#pragma unroll
+    for (int rho = 0; rho <= K; ++rho) {
+      float xr[M];
+      const int off = (tr + rho) * LW + tc + (RH - R);
+#pragma unroll
+      for (int m = 0; m < M; ++m) xr[m] = xs[off + m];
 #pragma unroll
       for (int dx = 0; dx < K; ++dx) {
-        if constexpr (DX) {
-          const float wv = grad_weight(a, (K - 1 - dy) * K + (K - 1 - dx));
-#pragma unroll
-          for (int j = 0; j < 4; ++j) gx[j] = __builtin_fmaf(wv, gr[j + dx], gx[j]);
-        }
-        if constexpr (DW) {
+        if (rho < K) {
 #pragma unroll
           for (int j = 0; j < 4; ++j)
-            dw[DW ? dy * K + dx : 0] =
-                __builtin_fmaf(gc[j], xr[DW ? j + dx : 0], dw[DW ? dy * K + dx : 0]);
+            dw[rho * K + dx] = __builtin_fmaf(g0[j], xr[j + dx], dw[rho * K + dx]);
+        }
+        if (rho >= 1) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            dw[(rho - 1) * K + dx] = __builtin_fmaf(g1[j], xr[j + dx], dw[(rho - 1) * K + dx]);
         }
       }
     }
-
-    if constexpr (DX) {
-      const int qy = y0 + tr;
-      if (qy < a.h) {
-        float* GX = a.gx + b * a.gx_plane + (long long)qy * a.gx_stride;
-        if (interior) {
-          *reinterpret_cast<float4*>(GX + x0 + tc) = make_float4(gx[0], gx[1], gx[2], gx[3]);
-        } else {
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const int qx = x0 + tc + j;
-            if (qx >= a.w) continue;
-            float v = gx[j];
-            if (a.boundary != HIPACC_BOUNDARY_CONSTANT &&
-                (qx < R || qx >= a.w - R || qy < R || qy >= a.h - R))
-              v += conv_fold(a, G, qx, qy, true);
-            GX[qx] = v;
-          }
-        }
-      }
-    }
+    cur = nxt;
   }
 
-  if constexpr (DW) {
-    const int lane = tid & 63, wave = tid >> 6;
+  const int lane = tid & 63, wave = tid >> 6;
 #pragma unroll
-    for (int t = 0; t < (DW ? KK : 1); ++t) {
-      const float v = wave_sum(dw[t]);
-      if (lane == 0) red[wave * KK + t] = v;
-    }
-    __syncthreads();
-    for (int t = tid; t < KK; t += 256)
-      a.partials[(long long)blockIdx.x * KK + t] =
-          ((red[t] + red[KK + t]) + red[2 * KK + t]) + red[3 * KK + t];
+  for (int k = 0; k < KK; ++k) {
+    const float v = wave_sum(dw[k]);
+    if (lane == 0) red[wave * KK + k] = v;
   }
-}
-
-// Any mask shape: one thread per dX pixel, full preimage sum.
-__global__ __launch_bounds__(256) void conv_grad_dx_generic(const ConvGradArgs a) {
-  const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y, b = blockIdx.z;
-  if (x >= a.w) return;
-  const float* G = a.gy + b * a.gy_plane;
-  a.gx[b * a.gx_plane + (long long)y * a.gx_stride + x] = conv_fold(a, G, x, y, false);
+  __syncthreads();
+  for (int k = tid; k < KK; k += 256)
+    a.partials[(long long)blockIdx.x * KK + k] =
+        ((red[k] + red[KK + k]) + red[2 * KK + k]) + red[3 * KK + k];
 }
 
 // Any mask shape: block (chunk, tap) sums dY(p) * X_bh(p + t - r) over the
 // pixels p = chunk*256 + tid + k*nchunks*256 (all images).
-__global__ __launch_bounds__(256) void conv_grad_dw_generic(const ConvGradArgs a) {
+__global__ __launch_bounds__(256) void conv_dw_generic(const ConvGradArgs a) {
   __shared__ float red[4];
   const int kk = a.kx * a.ky, t = blockIdx.y;
   const int ty = t / a.kx, tx = t - ty * a.kx, rx = a.kx / 2, ry = a.ky / 2;
@@ -222,8 +290,8 @@ __global__ __launch_bounds__(256) void conv_grad_dw_generic(const ConvGradArgs a
 }
 
 // dw[t] = SUM_b partials[b][t] in a fixed order.
-__global__ __launch_bounds__(256) void conv_grad_dw_reduce(const float* partials, int nblocks,
-                                                           int kk, float* dw) {
+__global__ __launch_bounds__(256) void conv_dw_reduce(const float* partials, int nblocks, int kk,
+                                                      float* dw) {
   __shared__ float red[4];
   const int t = blockIdx.x;
   float s = 0.0f;
@@ -234,63 +302,33 @@ __global__ __launch_bounds__(256) void conv_grad_dw_reduce(const float* partials
   if (threadIdx.x == 0) dw[t] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
-template <int R>
-static int launch_conv_grad_fast(const ConvGradArgs& a, bool dx, bool dw, int nblocks,
-                                 hipStream_t s) {
-  // dX and dw run as two launches: fused, the K*K dw accumulators and the dX
-  // windows together exceed the register file (occupancy 1); apart each
-  // keeps 4 waves per SIMD, and the second pass re-reads dY from cache.
-  if (dx) {
-    hipLaunchKernelGGL((conv_grad_kernel<R, true, false>), dim3(nblocks), dim3(256), 0, s, a);
-    int st = hip_status(hipGetLastError(), "conv_grad_kernel launch");
-    if (st) return st;
-  }
-  if (dw) {
-    hipLaunchKernelGGL((conv_grad_kernel<R, false, true>), dim3(nblocks), dim3(256), 0, s, a);
-    return hip_status(hipGetLastError(), "conv_grad_kernel launch");
-  }
-  return HIPACC_OK;
-}
-
-int launch_conv_grad(const ConvGradArgs& a0, float* dw, hipStream_t s) {
+int launch_conv_dw(const ConvGradArgs& a0, float* dw, hipStream_t s) {
   ConvGradArgs a = a0;
-  const bool dx = a.gx != nullptr, dwn = dw != nullptr;
-  if (!dx && !dwn) return HIPACC_OK;
   const int kk = a.kx * a.ky;
-  int nblocks = 0;
-  int st = HIPACC_OK;
+  int nblocks;
   if (a.kx == a.ky && (a.kx == 3 || a.kx == 5 || a.kx == 7 || a.kx == 9)) {
     a.tiles_x = ceil_div(a.w, 64);
-    a.tiles_y = ceil_div(a.h, 16);
+    a.tiles_y = ceil_div(a.h, 32);
     a.ntiles = a.tiles_x * a.tiles_y * a.batch;
     nblocks = a.ntiles < kGradBlocks ? a.ntiles : kGradBlocks;
+    const int per = ceil_div(a.ntiles, nblocks);
+    nblocks = ceil_div(a.ntiles, per);
     switch (a.kx) {
-      case 3: st = launch_conv_grad_fast<1>(a, dx, dwn, nblocks, s); break;
-      case 5: st = launch_conv_grad_fast<2>(a, dx, dwn, nblocks, s); break;
-      case 7: st = launch_conv_grad_fast<3>(a, dx, dwn, nblocks, s); break;
-      default: st = launch_conv_grad_fast<4>(a, dx, dwn, nblocks, s); break;
+      case 3: hipLaunchKernelGGL(conv_dw_kernel<1>, dim3(nblocks), dim3(256), 0, s, a, per); break;
+      case 5: hipLaunchKernelGGL(conv_dw_kernel<2>, dim3(nblocks), dim3(256), 0, s, a, per); break;
+      case 7: hipLaunchKernelGGL(conv_dw_kernel<3>, dim3(nblocks), dim3(256), 0, s, a, per); break;
+      default: hipLaunchKernelGGL(conv_dw_kernel<4>, dim3(nblocks), dim3(256), 0, s, a, per); break;
     }
   } else {
-    if (dx) {
-      hipLaunchKernelGGL(conv_grad_dx_generic, dim3(ceil_div(a.w, 256), a.h, a.batch), dim3(256),
-                         0, s, a);
-      st = hip_status(hipGetLastError(), "conv_grad_dx_generic launch");
-    }
-    if (st == HIPACC_OK && dwn) {
-      const long long total = (long long)a.w * a.h * a.batch;
-      long long nc = (total + 256 * 16 - 1) / (256 * 16);
-      const int cap = 256;
-      nblocks = (int)(nc < cap ? nc : cap);
-      hipLaunchKernelGGL(conv_grad_dw_generic, dim3(nblocks, kk), dim3(256), 0, s, a);
-      st = hip_status(hipGetLastError(), "conv_grad_dw_generic launch");
-    }
+    const long long total = (long long)a.w * a.h * a.batch;
+    const long long nc = (total + 256 * 16 - 1) / (256 * 16);
+    nblocks = (int)(nc < 256 ? nc : 256);
+    hipLaunchKernelGGL(conv_dw_generic, dim3(nblocks, kk), dim3(256), 0, s, a);
   }
-  if (st == HIPACC_OK && dwn) {
-    hipLaunchKernelGGL(conv_grad_dw_reduce, dim3(kk), dim3(256), 0, s, a.partials, nblocks, kk,
-                       dw);
-    st = hip_status(hipGetLastError(), "conv_grad_dw_reduce launch");
-  }
-  return st;
+  int st = hip_status(hipGetLastError(), "conv_dw launch");
+  if (st) return st;
+  hipLaunchKernelGGL(conv_dw_reduce, dim3(kk), dim3(256), 0, s, a.partials, nblocks, kk, dw);
+  return hip_status(hipGetLastError(), "conv_dw_reduce launch");
 }
 
 }  // namespace hh

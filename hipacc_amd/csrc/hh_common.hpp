@@ -8,6 +8,9 @@
 #include "../../include/hipacc_hip.h"
 
 #define HH_HD __host__ __device__ __forceinline__
+// Pointer qualifier for read-only device data read with uniform indices: the
+// AMDGPU constant address space, loaded with scalar (SMEM) loads.
+#define HH_CONST __attribute__((address_space(4)))
 
 namespace hh {
 

@@ -28,6 +28,73 @@ __device__ __forceinline__ float wave_sum(float v) {
 }
 
 
+// The ring of extended positions around a w x h image that boundary
+// handling folds back onto it: rows [-ry, 0) and [h, h+ry) over columns
+// [-rx, w+rx), and columns [-rx, 0) and [w, w+rx) over rows [0, h).
+HH_HD long long ring_size(int w, int h, int rx, int ry) {
+  return 2LL * ry * (w + 2 * rx) + 2LL * h * rx;
+}
+HH_HD long long ring_index(int w, int h, int rx, int ry, int ix, int iy) {
+  const long long ew = w + 2 * rx;
+  if (iy < 0) return (long long)(iy + ry) * ew + (ix + rx);
+  if (iy >= h) return ry * ew + (long long)(iy - h) * ew + (ix + rx);
+  const long long base = 2LL * ry * ew;
+  if (ix < 0) return base + (long long)iy * rx + (ix + rx);
+  return base + (long long)h * rx + (long long)iy * rx + (ix - w);
+}
+HH_HD void ring_pos(int w, int h, int rx, int ry, long long j, int& ix, int& iy) {
+  const long long ew = w + 2 * rx;
+  if (j < 2LL * ry * ew) {
+    const long long r = j / ew;
+    ix = (int)(j - r * ew) - rx;
+    iy = r < ry ? (int)r - ry : h + (int)(r - ry);
+    return;
+  }
+  j -= 2LL * ry * ew;
+  const long long side = (long long)h * rx;
+  const bool right = j >= side;
+  if (right) j -= side;
+  iy = (int)(j / rx);
+  const int c = (int)(j - (long long)iy * rx);
+  ix = right ? w + c : c - rx;
+}
+
+// Pixels within (rx, ry) of an edge: full rows at the top and bottom, the
+// side columns in between.
+HH_HD void band_rows(int w, int h, int rx, int ry, int& top, int& bot0, int& side) {
+  top = ry < h ? ry : h;
+  bot0 = h - ry > top ? h - ry : top;
+  side = rx < w ? rx : w;
+}
+HH_HD long long band_size(int w, int h, int rx, int ry) {
+  int top, bot0, side;
+  band_rows(w, h, rx, ry, top, bot0, side);
+  const int lcols = side, rcols = w - (w - rx > side ? w - rx : side);
+  return (long long)(top + (h - bot0)) * w + (long long)(bot0 - top) * (lcols + rcols);
+}
+HH_HD void band_pos(int w, int h, int rx, int ry, long long j, int& x, int& y) {
+  int top, bot0, side;
+  band_rows(w, h, rx, ry, top, bot0, side);
+  const long long full = (long long)(top + (h - bot0)) * w;
+  if (j < full) {
+    const int r = (int)(j / w);
+    x = (int)(j - (long long)r * w);
+    y = r < top ? r : bot0 + (r - top);
+    return;
+  }
+  j -= full;
+  const int rstart = w - rx > side ? w - rx : side;
+  const int lcols = side, rcols = w - rstart, per = lcols + rcols;
+  y = top + (int)(j / per);
+  const int c = (int)(j - (long long)(y - top) * per);
+  x = c < lcols ? c : rstart + (c - lcols);
+}
+
+// Adds, for every band pixel q, the ring values of the extended positions
+// (other than q) its boundary mode maps onto q, in a fixed order.
+int launch_fold_add(float* gx, long long gx_plane, int gx_stride, int w, int h, int batch, int rx,
+                    int ry, int mode, const float* ring, hipStream_t s);
+
 // Adjoint of out(p) = SUM_t w[t] * X_bh(p + t - r) over whole images (the
 // accessor window is the image, the iteration space has its size).
 struct ConvGradArgs {
@@ -47,9 +114,14 @@ struct ConvGradArgs {
   int tiles_x, tiles_y, ntiles;
 };
 
-// Launches the dX / dW kernels and, when dw != NULL, the final reduction of
-// the partials into dw (device, kx*ky floats).
-int launch_conv_grad(const ConvGradArgs& a, float* dw, hipStream_t s);
+// E(i) = SUM_t w[t] * dY0(i + r - t) (dY0: dY inside the image, 0 outside)
+// at every ring position i, into ring[b * ring_size + ring_index(i)].
+int launch_conv_ring(const ConvGradArgs& a, float* ring, hipStream_t s);
+// dw (device, kx*ky floats) via per-block partials in a.partials and a
+// fixed-order reduction.
+int launch_conv_dw(const ConvGradArgs& a, float* dw, hipStream_t s);
+// out[t] = w[n-1-t] (the flipped mask of the dX correlation).
+int launch_flip(const float* w, float* out, int n, hipStream_t s);
 
 // Backward of the bilateral filter (hipacc_hip_bilateral_grad).
 struct BilatGradArgs {
@@ -71,7 +143,10 @@ struct BilatGradArgs {
   int vec_ok;
   int tiles_x, tiles_y, ntiles;
 };
-// dsigma (device, 1 float) = -1/sigma_r^3 * dL/dc when not NULL.
-int launch_bilateral_grad(const BilatGradArgs& a, float* dsigma, float sigma_r, hipStream_t s);
+// dsigma (device, 1 float) = -1/sigma_r^3 * dL/dc when not NULL; ring
+// (when not NULL) receives the folded neighbour terms of the ring positions
+// and is added onto the band pixels of dX.
+int launch_bilateral_grad(const BilatGradArgs& a, float* dsigma, float sigma_r, float* ring,
+                          hipStream_t s);
 
 }  // namespace hh

@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void local_sum_kernel(const LocalArgs a) {
       for (int dx = 0; dx < MW; ++dx) {
         ACC w;
         if constexpr (WDEV)
-          w = (ACC)a.wdev[dy * MW + dx];
+          w = (ACC)((const HH_CONST float*)a.wdev)[dy * MW + dx];
         else if constexpr (std::is_same<ACC, float>::value)
           w = a.w.f[dy * MW + dx];
         else

@@ -97,7 +97,10 @@ template <typename ArgsT> struct StreamTaps;
 template <> struct StreamTaps<LocalArgs> {
   template <typename ACC, bool WDEV>
   __device__ __forceinline__ static ACC w2d(const LocalArgs& a, int t) {
-    if constexpr (WDEV) return (ACC)a.wdev[t];
+    // Device weights are read through the constant address space: uniform
+    // scalar loads into SGPRs (a generic pointer would make them per-lane
+    // vector loads held in VGPRs, since the output stores might alias).
+    if constexpr (WDEV) return (ACC)((const HH_CONST float*)a.wdev)[t];
     else if constexpr (std::is_same<ACC, float>::value) return a.w.f[t];
     else return (ACC)a.w.i[t];
   }
@@ -153,13 +156,14 @@ __device__ __forceinline__ void edge_row_load(TI (&v)[V], const Geom& g, const T
 // the row into the accumulators of the KY output rows it belongs to.  `slot0`
 // is the accumulator of output row j (= j mod KY); `cst_row` (separable,
 // CONSTANT) makes the intermediate row the constant.  Only output rows j-dyi
-// with dyi <= maxdy exist (maxdy < KY-1 in the first rows of a segment; it
-// is a constant after unrolling, so the skipped taps cost nothing).
+// with mindy <= dyi <= maxdy exist: maxdy < KY-1 in the first rows of a
+// segment (a constant after unrolling, so the skipped taps cost nothing),
+// mindy > 0 in its last rows, past the segment's last output.
 template <typename TI, typename ACC, int V, int KX, int KY, bool FMA, int KIND, bool WDEV,
           typename ArgsT>
 __device__ __forceinline__ void stream_accumulate(const ArgsT& a, const TI (&raw)[V],
                                                   ACC (&acc)[KY][V], int slot0, bool cst_row,
-                                                  int maxdy) {
+                                                  int maxdy, int mindy) {
   constexpr int RX = KX / 2;
   constexpr int NWIN = V + 2 * RX;
   constexpr bool IS_F = std::is_same<ACC, float>::value;
@@ -178,6 +182,7 @@ __device__ __forceinline__ void stream_accumulate(const ArgsT& a, const TI (&raw
 #pragma unroll
     for (int dyi = 0; dyi < KY; ++dyi) {
       if (dyi > maxdy) break;
+      if (dyi < mindy) continue;
       const int s = (slot0 - dyi + KY) % KY;
 #pragma unroll
       for (int dx = 0; dx < KX; ++dx) {
@@ -220,6 +225,7 @@ __device__ __forceinline__ void stream_accumulate(const ArgsT& a, const TI (&raw
 #pragma unroll
     for (int dyi = 0; dyi < KY; ++dyi) {
       if (dyi > maxdy) break;
+      if (dyi < mindy) continue;
       const int s = (slot0 - dyi + KY) % KY;
       if constexpr (KIND == kStreamBox) {
 #pragma unroll
@@ -267,13 +273,14 @@ __device__ __forceinline__ void stream_store(TO* __restrict__ orow, const Geom& 
 // accumulator slot are compile-time constants inside a block).  WARM: the
 // first block of the segment, whose row u only reaches outputs 0..u.
 template <typename TI, typename TO, typename ACC, int V, int KX, int KY, int PF, bool FMA,
-          int KIND, bool WDEV, bool FAST, bool WARM, typename ArgsT>
+          int KIND, bool WDEV, bool FAST, bool WARM, bool TAIL, typename ArgsT>
 __device__ __forceinline__ void stream_block(const ArgsT& a, const StreamCtx<TI, TO, V>& c,
                                              TI (&ring)[PF][V], ACC (&acc)[KY][V], const TI* base,
                                              TO* orow, bool full, bool any, float rnd, int jb) {
   const Geom& g = a.g;
   const bool cmode_sep = KIND != kStream2D && !FAST && g.boundary == HIPACC_BOUNDARY_CONSTANT;
-  const bool full_blk = jb + 2 * PF <= c.nj;  // every refill of this block is in range
+  // every refill of this block is in range (always for the non-tail blocks)
+  const bool full_blk = !TAIL || jb + 2 * PF <= c.nj;
 #pragma unroll
   for (int u = 0; u < PF; ++u) {
     const int j = jb + u;
@@ -291,8 +298,13 @@ __device__ __forceinline__ void stream_block(const ArgsT& a, const StreamCtx<TI,
       const int iy = c.iy0 + j;
       cst_row = iy < g.lo_y || iy >= g.hi_y;
     }
+    // The taps of outputs past the segment are skipped for the larger masks,
+    // where they are a fifth of the work; for small masks the extra
+    // branches cost more than they save.
+    constexpr bool SKIP_TAIL = TAIL && KY >= 7;
     stream_accumulate<TI, ACC, V, KX, KY, FMA, KIND, WDEV>(a, raw, acc, u % KY, cst_row,
-                                                           WARM ? u : KY - 1);
+                                                           WARM ? u : KY - 1,
+                                                           SKIP_TAIL ? j - c.nj + KY : 0);
     if (!WARM || u >= KY - 1)
       stream_store<TO, ACC, V>(orow + (long long)j * g.out_stride, g, full, any, c.xl, rnd,
                                acc[(u % KY + 1) % KY]);
@@ -321,11 +333,15 @@ __device__ __forceinline__ void stream_rows(const ArgsT& a, const StreamCtx<TI, 
     if constexpr (FAST) load_px<TI, V>(ring[p], base + (long long)r * g.in_stride);
     else edge_row_load<TI, V>(ring[p], g, c.in, c.cols, c.iy0 + r);
   }
-  stream_block<TI, TO, ACC, V, KX, KY, PF, FMA, KIND, WDEV, FAST, true>(a, c, ring, acc, base,
-                                                                        orow, full, any, rnd, 0);
-  for (int jb = PF; jb < c.nj; jb += PF)
-    stream_block<TI, TO, ACC, V, KX, KY, PF, FMA, KIND, WDEV, FAST, false>(a, c, ring, acc, base,
-                                                                           orow, full, any, rnd, jb);
+  stream_block<TI, TO, ACC, V, KX, KY, PF, FMA, KIND, WDEV, FAST, true, true>(
+      a, c, ring, acc, base, orow, full, any, rnd, 0);
+  int jb = PF;
+  for (; jb + 2 * PF <= c.nj; jb += PF)
+    stream_block<TI, TO, ACC, V, KX, KY, PF, FMA, KIND, WDEV, FAST, false, false>(
+        a, c, ring, acc, base, orow, full, any, rnd, jb);
+  for (; jb < c.nj; jb += PF)
+    stream_block<TI, TO, ACC, V, KX, KY, PF, FMA, KIND, WDEV, FAST, false, true>(
+        a, c, ring, acc, base, orow, full, any, rnd, jb);
 }
 
 template <typename TI, typename TO, typename ACC, int V, int KX, int KY, int PF, bool FMA,

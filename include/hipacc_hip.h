@@ -274,8 +274,11 @@ int hipacc_hip_bilateral(const hipacc_bilateral_desc* desc, void* stream);
  * are clamped to the image, so it folds as CLAMP).  dw is a fixed-order
  * reduction (deterministic).  F32 images; in, grad_out and grad_in have the
  * same size and batch.  No counterpart in the reference (it has no adjoints).
- * workspace: NULL (the library's per-device buffer) or device memory of
- * hipacc_hip_grad_workspace_bytes() bytes.                                  */
+ * The flipped-mask stencil over the image runs through the forward's
+ * kernels (hipacc_hip_local, CONSTANT 0); the folded border terms are
+ * evaluated once per halo position and added to the border pixels.
+ * workspace: NULL (the library's per-device buffer, grown on demand) or
+ * device memory of hipacc_hip_local_grad_workspace_bytes(desc) bytes.       */
 typedef struct hipacc_local_grad_desc {
   hipacc_view in;            /* X, the forward input                        */
   hipacc_view grad_out;      /* dL/dY                                       */
@@ -289,7 +292,7 @@ typedef struct hipacc_local_grad_desc {
   double  const_val;
   void*   workspace;
 } hipacc_local_grad_desc;
-size_t hipacc_hip_grad_workspace_bytes(void);
+size_t hipacc_hip_local_grad_workspace_bytes(const hipacc_local_grad_desc* desc);
 int hipacc_hip_local_grad(const hipacc_local_grad_desc* desc, void* stream);
 
 /* ---- bilateral filter with a learnable range sigma -------------------- */
@@ -319,8 +322,9 @@ typedef struct hipacc_bilateral_grad_desc {
   int32_t boundary;
   double  const_val;
   float*  grad_sigma_r_dev;  /* device, 1 float, overwritten; NULL = skip   */
-  void*   workspace;         /* NULL or hipacc_hip_grad_workspace_bytes()   */
+  void*   workspace;         /* NULL or hipacc_hip_bilateral_grad_workspace_bytes() */
 } hipacc_bilateral_grad_desc;
+size_t hipacc_hip_bilateral_grad_workspace_bytes(const hipacc_bilateral_grad_desc* desc);
 int hipacc_hip_bilateral_train(const hipacc_bilateral_grad_desc* desc, void* stream);
 int hipacc_hip_bilateral_grad(const hipacc_bilateral_grad_desc* desc, void* stream);
 

@@ -126,3 +126,15 @@ def test_cpp_runtime_compiles_and_links(tmp_path):
                         "-o", str(exe), "-L", str(ROOT / "hipacc_amd"), "-lhipacc_gfx950"],
                        capture_output=True, text=True)
     assert p.returncode == 0, p.stderr
+
+
+def test_adjoint_ring_and_band_enumerations(tmp_path):
+    """The halo ring and border band the adjoint kernels enumerate (host
+    build of the same header functions)."""
+    exe = tmp_path / "ring_check"
+    p = subprocess.run(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O1", "-std=c++17",
+                        "-I", str(ROOT / "include"), str(ROOT / "tests" / "cpp" / "ring_check.hip"),
+                        "-o", str(exe)], capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr[-3000:]
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr
