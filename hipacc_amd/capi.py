@@ -133,6 +133,7 @@ SIGNATURES = {
     "hipacc_hip_pyramid_run_direct": (c_int32, [c_void_p, c_void_p]),
     "hipacc_hip_pyramid_plan_destroy": (c_int32, [c_void_p]),
     "hipacc_hip_local_grad": (c_int32, [POINTER(LocalGradDesc), c_void_p]),
+    "hipacc_hip_local_grad_variant": (c_int32, [POINTER(LocalGradDesc), c_int32, c_void_p]),
     "hipacc_hip_bilateral_train": (c_int32, [POINTER(BilateralGradDesc), c_void_p]),
     "hipacc_hip_bilateral_grad": (c_int32, [POINTER(BilateralGradDesc), c_void_p]),
     "hipacc_hip_alloc2d": (c_int32, [POINTER(c_void_p), POINTER(c_int32), c_int32, c_int32, c_int32,

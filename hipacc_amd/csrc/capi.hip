@@ -789,8 +789,15 @@ size_t hipacc_hip_local_grad_workspace_bytes(const hipacc_local_grad_desc* d) {
 }
 
 int hipacc_hip_local_grad(const hipacc_local_grad_desc* d, void* stream) {
+  return hipacc_hip_local_grad_variant(d, 0, stream);
+}
+
+int hipacc_hip_local_grad_variant(const hipacc_local_grad_desc* d, int32_t variant,
+                                  void* stream) {
   int st = check_local_grad(d);
   if (st) return st;
+  if (variant != kVarAuto && variant != kVarTile && variant != kVarStream)
+    return set_error(HIPACC_ERR_INVALID_ARG, "local_grad: variant %d", variant);
   const bool dx = d->grad_in.data != nullptr, dw = d->grad_weights_dev != nullptr;
   if (!dx && !dw) return HIPACC_OK;
   const int kx = d->mask_w, ky = d->mask_h, kk = kx * ky;
@@ -823,6 +830,7 @@ int hipacc_hip_local_grad(const hipacc_local_grad_desc* d, void* stream) {
   a.wdev = d->weights_dev;
   if (!a.wdev) std::memcpy(a.wh, d->weights_f32, sizeof(float) * kk);
   a.partials = reinterpret_cast<float*>(ws + lay.part);
+  a.dw_variant = variant;
   hipStream_t s = static_cast<hipStream_t>(stream);
   Timer tm(s);
   if (dx) {
@@ -945,6 +953,12 @@ int hipacc_hip_bilateral_grad(const hipacc_bilateral_grad_desc* d, void* stream)
   std::memset(&a, 0, sizeof(a));
   if ((st = bilat_common(d, a.sp))) return st;
   if (!dx && !ds) return HIPACC_OK;
+  const int kk = d->ksize * d->ksize;
+  a.sym = 1;
+  for (int t = 0; t < kk; ++t) {
+    a.lsp[t] = a.sp[t] > 0.0f ? std::log2(a.sp[t]) : -INFINITY;
+    a.sym = a.sym && a.sp[t] == a.sp[kk - 1 - t];
+  }
   a.x = static_cast<const float*>(d->in.data);
   a.y = static_cast<const float*>(d->out.data);
   a.d = static_cast<const float*>(d->norm.data);

@@ -54,7 +54,10 @@ __global__ __launch_bounds__(256) void bilat_ring_kernel(const BilatGradArgs a, 
 // 64 x 16 pixel tiles, 4 adjacent pixels per lane.  LDS holds, with a halo of
 // r rows and RH columns: X_bh (CONSTANT: the constant), a = dY/D (0 outside
 // the image) and Y.
-template <int R, bool DX>
+// SYM: the spatial table is point-symmetric (sp[t] == sp[K*K-1-t], as the
+// sigma_d table is), so the weight of a pixel pair is one exponential
+// exp2(k2*d^2 + log2 sp) for both its centre and its neighbour term.
+template <int R, bool DX, bool SYM>
 __global__ __launch_bounds__(256) void bilat_grad_kernel(const BilatGradArgs a) {
   constexpr int K = 2 * R + 1, KK = K * K;
   constexpr int TW = 64, TH = 16;
@@ -147,16 +150,24 @@ __global__ __launch_bounds__(256) void bilat_grad_kernel(const BilatGradArgs a) 
       for (int dx = 0; dx < K; ++dx) {
         const float spc = a.sp[dy * K + dx];           // tap o of q
         const float spn = a.sp[KK - 1 - (dy * K + dx)]; // tap -o of p = q + o
+        const float lsp = a.lsp[dy * K + dx];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const float v = xr[j + dx];
           const float d = v - xq[j];
-          const float e = __builtin_amdgcn_exp2f(k2 * (d * d));
-          const float u = (e * spc) * d * (v - yq[j]);
+          float sc, sn;   // s of the pair for the centre and the neighbour term
+          if constexpr (SYM) {
+            sc = sn = __builtin_amdgcn_exp2f(__builtin_fmaf(k2, d * d, lsp));
+          } else {
+            const float e = __builtin_amdgcn_exp2f(k2 * (d * d));
+            sc = e * spc;
+            sn = e * spn;
+          }
+          const float u = (sc * d) * (v - yq[j]);
           accc[j] += u;
           accs[j] = __builtin_fmaf(u, d, accs[j]);
           if constexpr (DX) {
-            const float w = ar[j + dx] * (e * spn);
+            const float w = ar[j + dx] * sn;
             accn[j] += w;
             accm[j] = __builtin_fmaf(w * d, xq[j] - yr[j + dx], accm[j]);
           }
@@ -208,10 +219,16 @@ __global__ __launch_bounds__(256) void bilat_grad_reduce(const float* partials, 
 
 template <int R>
 static int launch_bg(const BilatGradArgs& a, int nblocks, hipStream_t s) {
-  if (a.gx)
-    hipLaunchKernelGGL((bilat_grad_kernel<R, true>), dim3(nblocks), dim3(256), 0, s, a);
-  else
-    hipLaunchKernelGGL((bilat_grad_kernel<R, false>), dim3(nblocks), dim3(256), 0, s, a);
+  if (a.sym) {
+    if (a.gx)
+      hipLaunchKernelGGL((bilat_grad_kernel<R, true, true>), dim3(nblocks), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((bilat_grad_kernel<R, false, true>), dim3(nblocks), dim3(256), 0, s, a);
+  } else if (a.gx) {
+    hipLaunchKernelGGL((bilat_grad_kernel<R, true, false>), dim3(nblocks), dim3(256), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((bilat_grad_kernel<R, false, false>), dim3(nblocks), dim3(256), 0, s, a);
+  }
   return hip_status(hipGetLastError(), "bilat_grad_kernel launch");
 }
 

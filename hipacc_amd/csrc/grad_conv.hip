@@ -15,6 +15,7 @@
 // ring position here and added to the border band.  dw: per-block partial
 // sums, then a fixed-order reduction (deterministic run to run).
 #include "hh_grad.hpp"
+#include "hh_stream.hpp"
 
 namespace hh {
 
@@ -255,6 +256,142 @@ __global__ __launch_bounds__(256) void conv_dw_kernel(const ConvGradArgs a, int 
         ((red[k] + red[KK + k]) + red[2 * KK + k]) + red[3 * KK + k];
 }
 
+// dw by row streaming (no LDS tiles): a wave owns a strip of 62 lanes x 4
+// columns of dY and a segment of S rows; lanes 0 and 63 only supply the X
+// halo.  The wave walks the X rows the segment's windows touch (S + 2R rows)
+// and keeps the K dY rows each X row pairs with in a shifting register ring:
+// X row iy meets dY row iy + R - dy through tap row dy.  The X window of a
+// lane comes from its neighbours through DPP shifts.  Waves that touch the
+// image border read X through the boundary map (per-lane column tables,
+// scalar row remap).
+template <int R, int S>
+__global__ __launch_bounds__(256) void conv_dw_stream_kernel(const ConvGradArgs a, int strips,
+                                                             int segs, int total_waves) {
+  constexpr int K = 2 * R + 1, KK = K * K, V = 4;
+  constexpr int HL = (R + V - 1) / V;
+  constexpr int NW = V + 2 * R;
+  constexpr int STRIP = (64 - 2 * HL) * V;
+  __shared__ float red[4 * KK];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float dw[KK];
+#pragma unroll
+  for (int t = 0; t < KK; ++t) dw[t] = 0.0f;
+
+  for (int wv = blockIdx.x * 4 + wave; wv < total_waves; wv += gridDim.x * 4) {
+    const int per_img = strips * segs;
+    const int b = wv / per_img;
+    const int rem = wv - b * per_img;
+    const int strip = rem / segs, seg = rem - strip * segs;
+    const float* X = a.x + b * a.x_plane;
+    const float* G = a.gy + b * a.gy_plane;
+    const int x_wave = strip * STRIP - HL * V;        // column of lane 0's pixel 0
+    const int xl = x_wave + lane * V;
+    const bool own = lane >= HL && lane < 64 - HL;
+    const int y0 = seg * S;
+    const int ny = min(S, a.h - y0);
+    const int nx = ny + 2 * R;                         // X rows streamed
+    const bool fast = a.vec_ok && x_wave >= 0 && x_wave + 64 * V <= a.w && y0 - R >= 0 &&
+                      y0 + ny + R <= a.h;
+    Geom g;
+    g.in = X;
+    g.in_stride = a.x_stride;
+    g.lo_x = 0;
+    g.hi_x = a.w;
+    g.lo_y = 0;
+    g.hi_y = a.h;
+    g.in_imgw = a.w;
+    g.in_imgh = a.h;
+    g.boundary = a.boundary;
+    g.const_f = a.cval;
+    g.const_i = 0;
+    g.in_dtype = HIPACC_F32;
+    const LaneCols<V> cols = lane_cols<V>(g, xl);
+
+    auto load_x = [&](float (&v)[V], int k) {
+      const int iy = y0 - R + k;
+      if (fast) {
+        load_px<float, V>(v, X + (long long)iy * a.x_stride + xl);
+      } else {
+        bool rc;
+        const int y = row_src(g, iy, rc);
+        const float* row = X + (long long)y * a.x_stride;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          const float t = row[cols.x[e]];
+          v[e] = (rc || cols.cst[e]) ? a.cval : t;
+        }
+      }
+    };
+    auto load_g = [&](float (&v)[V], int k) {
+      const int y = y0 + k;
+#pragma unroll
+      for (int e = 0; e < V; ++e) v[e] = 0.0f;
+      if (k >= ny || !own) return;
+      const float* row = G + (long long)y * a.gy_stride;
+      if (fast) {
+        load_px<float, V>(v, row + xl);
+      } else {
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+          if (xl + e < a.w) v[e] = row[xl + e];
+      }
+    };
+
+    float ring[K][V];
+#pragma unroll
+    for (int d = 0; d < K; ++d)
+#pragma unroll
+      for (int e = 0; e < V; ++e) ring[d][e] = 0.0f;
+    float xn[V], gn[V];
+    load_x(xn, 0);
+    load_g(gn, 0);
+    for (int k = 0; k < nx; ++k) {
+      float cur[V];
+#pragma unroll
+      for (int e = 0; e < V; ++e) cur[e] = xn[e];
+#pragma unroll
+      for (int d = K - 1; d > 0; --d)
+#pragma unroll
+        for (int e = 0; e < V; ++e) ring[d][e] = ring[d - 1][e];
+#pragma unroll
+      for (int e = 0; e < V; ++e) ring[0][e] = gn[e];
+      if (k + 1 < nx) {
+        load_x(xn, k + 1);
+        load_g(gn, k + 1);
+      }
+      float win[NW];
+#pragma unroll
+      for (int wi = 0; wi < NW; ++wi) {
+        const int rel = wi - R;
+        const int dd = rel >= 0 ? rel / V : -((-rel + V - 1) / V);
+        const int e = rel - dd * V;
+        win[wi] = dd == 0 ? cur[e] : lane_shift(cur[e], dd);
+      }
+      // dY row y0 + k - dy exists for k - ny < dy <= k
+      const int dmin = k - ny + 1, dmax = k;
+#pragma unroll
+      for (int dy = 0; dy < K; ++dy) {
+        if (dy < dmin || dy > dmax) continue;
+#pragma unroll
+        for (int dx = 0; dx < K; ++dx)
+#pragma unroll
+          for (int j = 0; j < V; ++j)
+            dw[dy * K + dx] = __builtin_fmaf(ring[dy][j], win[j + dx], dw[dy * K + dx]);
+      }
+    }
+  }
+
+#pragma unroll
+  for (int t = 0; t < KK; ++t) {
+    const float v = wave_sum(dw[t]);
+    if (lane == 0) red[wave * KK + t] = v;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < KK; t += 256)
+    a.partials[(long long)blockIdx.x * KK + t] =
+        ((red[t] + red[KK + t]) + red[2 * KK + t]) + red[3 * KK + t];
+}
+
 // Any mask shape: block (chunk, tap) sums dY(p) * X_bh(p + t - r) over the
 // pixels p = chunk*256 + tid + k*nchunks*256 (all images).
 __global__ __launch_bounds__(256) void conv_dw_generic(const ConvGradArgs a) {
@@ -306,7 +443,19 @@ int launch_conv_dw(const ConvGradArgs& a0, float* dw, hipStream_t s) {
   ConvGradArgs a = a0;
   const int kk = a.kx * a.ky;
   int nblocks;
-  if (a.kx == a.ky && (a.kx == 3 || a.kx == 5 || a.kx == 7 || a.kx == 9)) {
+  if (a.kx == a.ky && (a.kx == 3 || a.kx == 5 || a.kx == 7 || a.kx == 9) && a.dw_variant != kVarTile) {
+    constexpr int S = 16;
+    const int strip = (64 - 2) * 4;   // HL = 1 for R <= 4
+    const int strips = ceil_div(a.w, strip), segs = ceil_div(a.h, S);
+    const int waves = strips * segs * a.batch;
+    nblocks = min(ceil_div(waves, 4), kGradBlocks);
+    switch (a.kx) {
+      case 3: hipLaunchKernelGGL((conv_dw_stream_kernel<1, S>), dim3(nblocks), dim3(256), 0, s, a, strips, segs, waves); break;
+      case 5: hipLaunchKernelGGL((conv_dw_stream_kernel<2, S>), dim3(nblocks), dim3(256), 0, s, a, strips, segs, waves); break;
+      case 7: hipLaunchKernelGGL((conv_dw_stream_kernel<3, S>), dim3(nblocks), dim3(256), 0, s, a, strips, segs, waves); break;
+      default: hipLaunchKernelGGL((conv_dw_stream_kernel<4, S>), dim3(nblocks), dim3(256), 0, s, a, strips, segs, waves); break;
+    }
+  } else if (a.kx == a.ky && (a.kx == 3 || a.kx == 5 || a.kx == 7 || a.kx == 9)) {
     a.tiles_x = ceil_div(a.w, 64);
     a.tiles_y = ceil_div(a.h, 32);
     a.ntiles = a.tiles_x * a.tiles_y * a.batch;

@@ -112,6 +112,7 @@ struct ConvGradArgs {
   float wh[kMaxTaps];
   float* partials;     // [blocks][kx*ky] weight-gradient partial sums
   int tiles_x, tiles_y, ntiles;
+  int dw_variant;      // kVarTile: LDS tiles; otherwise row streaming
 };
 
 // E(i) = SUM_t w[t] * dY0(i + r - t) (dY0: dY inside the image, 0 outside)
@@ -139,6 +140,8 @@ struct BilatGradArgs {
   float c;             // 0.5 / sigma_r^2
   float k2;            // -c * log2(e)
   float sp[kMaxTaps];  // spatial weights
+  float lsp[kMaxTaps]; // log2 of the spatial weights
+  int sym;             // sp is point-symmetric
   float* partials;     // [blocks] partial sums of dL/dc
   int vec_ok;
   int tiles_x, tiles_y, ntiles;

@@ -299,7 +299,7 @@ __device__ __forceinline__ void stream_block(const ArgsT& a, const StreamCtx<TI,
       cst_row = iy < g.lo_y || iy >= g.hi_y;
     }
     // The taps of outputs past the segment are skipped for the larger masks,
-    // where they are a fifth of the work; for small masks the extra
+    // where they are a fifth of the work or more; for small masks the extra
     // branches cost more than they save.
     constexpr bool SKIP_TAIL = TAIL && KY >= 7;
     stream_accumulate<TI, ACC, V, KX, KY, FMA, KIND, WDEV>(a, raw, acc, u % KY, cst_row,
@@ -335,13 +335,20 @@ __device__ __forceinline__ void stream_rows(const ArgsT& a, const StreamCtx<TI, 
   }
   stream_block<TI, TO, ACC, V, KX, KY, PF, FMA, KIND, WDEV, FAST, true, true>(
       a, c, ring, acc, base, orow, full, any, rnd, 0);
-  int jb = PF;
-  for (; jb + 2 * PF <= c.nj; jb += PF)
-    stream_block<TI, TO, ACC, V, KX, KY, PF, FMA, KIND, WDEV, FAST, false, false>(
-        a, c, ring, acc, base, orow, full, any, rnd, jb);
-  for (; jb < c.nj; jb += PF)
-    stream_block<TI, TO, ACC, V, KX, KY, PF, FMA, KIND, WDEV, FAST, false, true>(
-        a, c, ring, acc, base, orow, full, any, rnd, jb);
+  if constexpr (KY >= 7) {
+    // full blocks, then the tail blocks that skip the taps past the segment
+    int jb = PF;
+    for (; jb + 2 * PF <= c.nj; jb += PF)
+      stream_block<TI, TO, ACC, V, KX, KY, PF, FMA, KIND, WDEV, FAST, false, false>(
+          a, c, ring, acc, base, orow, full, any, rnd, jb);
+    for (; jb < c.nj; jb += PF)
+      stream_block<TI, TO, ACC, V, KX, KY, PF, FMA, KIND, WDEV, FAST, false, true>(
+          a, c, ring, acc, base, orow, full, any, rnd, jb);
+  } else {
+    for (int jb = PF; jb < c.nj; jb += PF)
+      stream_block<TI, TO, ACC, V, KX, KY, PF, FMA, KIND, WDEV, FAST, false, true>(
+          a, c, ring, acc, base, orow, full, any, rnd, jb);
+  }
 }
 
 template <typename TI, typename TO, typename ACC, int V, int KX, int KY, int PF, bool FMA,

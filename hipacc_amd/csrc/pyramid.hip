@@ -68,15 +68,20 @@ __device__ __forceinline__ float pyr_read(const float* g, int stride, int w, int
   return g[(long long)y * stride + x];
 }
 
-// For each tile (TW x TH pixels of level l): the G[l+1] values the tile's
-// LF reads and the ones it owns (columns x0/2.., rows y0/2..) are computed
-// into LDS -- the 5x5 convolution of G[l] evaluated only at the NN sample
-// points -- the owned ones are stored, then L[l] = G[l] - LF(G[l+1]).
+// For each tile (TW x TH pixels of level l): the G[l] region the tile reads
+// is staged in LDS once (with the convolution's boundary mode); from it the
+// G[l+1] values the tile's LF reads and the ones it owns (columns x0/2..,
+// rows y0/2..) are computed -- the 5x5 convolution evaluated only at the NN
+// sample points -- the owned ones are stored, then L[l] = G[l] - LF(G[l+1]).
 template <int TH, bool FMA>
 __global__ __launch_bounds__(256) void pyramid_level_kernel(const PyrLevelArgs a) {
   constexpr int TW = 64;
   constexpr int UW = TW / 2 + 4, UH = TH / 2 + 4;   // G[l+1] region capacity
+  // G[l] region capacity: the NN map has step s = W/W1 <= 3, so the blur
+  // taps of UW x UH sample points span at most 3*(UW-1)+5 x 3*(UH-1)+5.
+  constexpr int RW = 3 * (UW - 1) + 5, RH = 3 * (UH - 1) + 5;
   __shared__ float g1s[UH * UW];
+  __shared__ float gls[RH * RW];
   const int l = a.level;
 
   for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
@@ -106,33 +111,41 @@ __global__ __launch_bounds__(256) void pyramid_level_kernel(const PyrLevelArgs a
     ux1 = max(ux1, ox1);
     uy1 = max(uy1, oy1);
     const int uw = ux1 - ux0 + 1, uh = uy1 - uy0 + 1;
+    // G[l] region of the blur taps at the NN sample points
+    const float sd = (float)W / (float)W1, td = (float)H / (float)H1;
+    const int rx0 = (int)interp_map(sd, ux0) - 2, rx1 = (int)interp_map(sd, ux1) + 2;
+    const int ry0 = (int)interp_map(td, uy0) - 2, ry1 = (int)interp_map(td, uy1) + 2;
+    const int rw = rx1 - rx0 + 1, rh = ry1 - ry0 + 1;
+    if (uw > UW || uh > UH || rw > RW || rh > RH) continue;   // cannot happen (see above)
 
     __syncthreads();  // previous tile's LDS reads are done
-    if (uw <= UW && uh <= UH) {
-      const float sd = (float)W / (float)W1, td = (float)H / (float)H1;
-      for (int i = threadIdx.x; i < uw * uh; i += 256) {
-        const int yy = i / uw, xx = i - yy * uw;
-        const int xu = ux0 + xx, yu = uy0 + yy;
-        const int nx = (int)interp_map(sd, xu), ny = (int)interp_map(td, yu);
-        float acc = 0.0f;
-#pragma unroll
-        for (int ty = 0; ty < 5; ++ty)
-#pragma unroll
-          for (int tx = 0; tx < 5; ++tx) {
-            const float v = pyr_read(G, gs, W, H, a.boundary, nx + tx - 2, ny + ty - 2);
-            const float wv = a.w[ty * 5 + tx];
-            if (ty == 0 && tx == 0)
-              acc = mul_first(wv, v);
-            else
-              acc = madd<FMA>(wv, v, acc);
-          }
-        g1s[yy * UW + xx] = acc;
-        if (xu >= ox0 && xu <= ox1 && yu >= oy0 && yu <= oy1)
-          G1[(long long)yu * g1st + xu] = acc;
-      }
+    for (int i = threadIdx.x; i < rw * rh; i += 256) {
+      const int r = i / rw, c = i - r * rw;
+      gls[r * RW + c] = pyr_read(G, gs, W, H, a.boundary, rx0 + c, ry0 + r);
     }
     __syncthreads();
-    if (uw > UW || uh > UH) continue;   // never for the level sizes of a pyramid
+    for (int i = threadIdx.x; i < uw * uh; i += 256) {
+      const int yy = i / uw, xx = i - yy * uw;
+      const int xu = ux0 + xx, yu = uy0 + yy;
+      const int nx = (int)interp_map(sd, xu) - 2 - rx0, ny = (int)interp_map(td, yu) - 2 - ry0;
+      const float* src = gls + ny * RW + nx;
+      float acc = 0.0f;
+#pragma unroll
+      for (int ty = 0; ty < 5; ++ty)
+#pragma unroll
+        for (int tx = 0; tx < 5; ++tx) {
+          const float v = src[ty * RW + tx];
+          const float wv = a.w[ty * 5 + tx];
+          if (ty == 0 && tx == 0)
+            acc = mul_first(wv, v);
+          else
+            acc = madd<FMA>(wv, v, acc);
+        }
+      g1s[yy * UW + xx] = acc;
+      if (xu >= ox0 && xu <= ox1 && yu >= oy0 && yu <= oy1)
+        G1[(long long)yu * g1st + xu] = acc;
+    }
+    __syncthreads();
 
     for (int i = threadIdx.x; i < TW * TH; i += 256) {
       const int r = i / TW, c = i - r * TW;
@@ -146,7 +159,7 @@ __global__ __launch_bounds__(256) void pyramid_level_kernel(const PyrLevelArgs a
       const int ya = yi - uy0, yb = min(yi + 1, H1 - 1) - uy0;
       const float lf = interp_bilinear(fx, fy, g1s[ya * UW + xa], g1s[ya * UW + xb],
                                        g1s[yb * UW + xa], g1s[yb * UW + xb]);
-      L[(long long)y * ls + x] = G[(long long)y * gs + x] - lf;
+      L[(long long)y * ls + x] = gls[(y - ry0) * RW + (x - rx0)] - lf;
     }
   }
 }

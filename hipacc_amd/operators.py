@@ -68,7 +68,8 @@ class Call:
     forces a kernel variant (capi.VAR_*) with its tile parameter."""
 
     _VARIANT_FN = {"hipacc_hip_local": "hipacc_hip_local_variant",
-                   "hipacc_hip_separable": "hipacc_hip_separable_variant"}
+                   "hipacc_hip_separable": "hipacc_hip_separable_variant",
+                   "hipacc_hip_local_grad": "hipacc_hip_local_grad_variant"}
     _TUNE_FN = {"hipacc_hip_local": "hipacc_hip_tune_local",
                 "hipacc_hip_separable": "hipacc_hip_tune_separable"}
 
@@ -80,7 +81,7 @@ class Call:
         self.variant, self.param = variant, param
         if variant:
             self.fn = getattr(capi.lib(), self._VARIANT_FN[fn_name])
-            self.extra = (variant, param)
+            self.extra = (variant,) if fn_name == "hipacc_hip_local_grad" else (variant, param)
         else:
             self.fn = getattr(capi.lib(), fn_name)
             self.extra = ()

@@ -294,6 +294,11 @@ typedef struct hipacc_local_grad_desc {
 } hipacc_local_grad_desc;
 size_t hipacc_hip_local_grad_workspace_bytes(const hipacc_local_grad_desc* desc);
 int hipacc_hip_local_grad(const hipacc_local_grad_desc* desc, void* stream);
+/* The dw kernel variant: 0 automatic, 1 LDS tiles, 2 row streaming (square
+ * masks up to 9x9; other shapes use a generic kernel).  All variants give
+ * the same dX; dw agrees to rounding (each sums in its own fixed order).    */
+int hipacc_hip_local_grad_variant(const hipacc_local_grad_desc* desc, int32_t variant,
+                                  void* stream);
 
 /* ---- bilateral filter with a learnable range sigma -------------------- */
 /* Training forward: hipacc_hip_bilateral (F32, whole images) that also

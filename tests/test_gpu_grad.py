@@ -57,6 +57,12 @@ def test_conv_grad_matches_oracle(gpu, mode, kw, kh, shape):
     op.prepare_local_grad(X, G, None, kw, kh, weights=w, grad_weights=GW2, boundary=mode,
                           const_val=cval).launch()
     assert torch.equal(GW, GW2), "dw differs between runs (must be deterministic)"
+    for variant in (capi.VAR_TILE, capi.VAR_STREAM):
+        GXv, GWv = torch.empty_like(X), torch.empty_like(GW)
+        op.prepare_local_grad(X, G, GXv, kw, kh, weights_dev=W, grad_weights=GWv, boundary=mode,
+                              const_val=cval).with_variant(variant).launch()
+        assert torch.equal(GXv, GX)
+        _within(to_host(GWv, np.float32), dw, dwa, f"dw variant {variant}")
 
 
 @pytest.mark.parametrize("mode", [capi.CLAMP, capi.MIRROR, capi.REPEAT])
