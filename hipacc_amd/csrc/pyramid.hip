@@ -147,19 +147,25 @@ __global__ __launch_bounds__(256) void pyramid_level_kernel(const PyrLevelArgs a
     }
     __syncthreads();
 
-    for (int i = threadIdx.x; i < TW * TH; i += 256) {
-      const int r = i / TW, c = i - r * TW;
-      const int x = x0 + c, y = y0 + r;
-      if (x >= W || y >= H) continue;
-      int xi, yi;
-      float fx, fy;
+    // a lane keeps one column (its LF column weights are computed once) and
+    // walks the rows tid/64, +4, ...
+    const int x = x0 + (threadIdx.x & 63);
+    if (x < W) {
+      int xi;
+      float fx;
       interp_lf(interp_map(su, x), xi, fx);
-      interp_lf(interp_map(sv, y), yi, fy);
       const int xa = xi - ux0, xb = min(xi + 1, W1 - 1) - ux0;
-      const int ya = yi - uy0, yb = min(yi + 1, H1 - 1) - uy0;
-      const float lf = interp_bilinear(fx, fy, g1s[ya * UW + xa], g1s[ya * UW + xb],
-                                       g1s[yb * UW + xa], g1s[yb * UW + xb]);
-      L[(long long)y * ls + x] = gls[(y - ry0) * RW + (x - rx0)] - lf;
+      for (int r = threadIdx.x >> 6; r < TH; r += 4) {
+        const int y = y0 + r;
+        if (y >= H) break;
+        int yi;
+        float fy;
+        interp_lf(interp_map(sv, y), yi, fy);
+        const int ya = yi - uy0, yb = min(yi + 1, H1 - 1) - uy0;
+        const float lf = interp_bilinear(fx, fy, g1s[ya * UW + xa], g1s[ya * UW + xb],
+                                         g1s[yb * UW + xa], g1s[yb * UW + xb]);
+        L[(long long)y * ls + x] = gls[(y - ry0) * RW + (x - rx0)] - lf;
+      }
     }
   }
 }
@@ -198,7 +204,9 @@ static int plan_launches(hipacc_pyramid_plan* p, hipStream_t s) {
     a.level = l;
     a.boundary = p->boundary;
     std::memcpy(a.w, p->w, sizeof(a.w));
-    const int nb = a.ntiles < 2048 ? a.ntiles : 2048;
+    // one block per tile: the hardware keeps several tiles per CU in flight,
+    // hiding the staging latency of one behind the arithmetic of another
+    const int nb = a.ntiles;
     int st;
     if (p->fma) {
       st = p->tile_rows == 8    ? launch_level_t<8, true>(a, nb, s)
