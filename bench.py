@@ -66,7 +66,7 @@ def time_calls(calls, steps, warmup):
     return e0.elapsed_time(e1) / steps
 
 
-HEAD_CANDIDATES = ("stream:8", "stream:12", "stream:16", "stream:24", "wide:16", "tile:4")
+HEAD_CANDIDATES = ("ring:0", "ring:48", "stream:12", "stream:16", "wide:16", "tile:4")
 
 
 def parse_variant(text):
@@ -302,7 +302,7 @@ def main() -> int:
     ap.add_argument("--profile", action="store_true",
                     help="launch the headline kernel 20 times and a 64 MiB copy 20 times, "
                          "for rocprofv3 counter runs; prints no result line")
-    ap.add_argument("--variant", default="stream:12",
+    ap.add_argument("--variant", default="ring:0",
                     help="headline kernel variant under --profile (name:param)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -360,8 +360,9 @@ def main() -> int:
                    "fp_contract": True},
         "roofline": {"bound": "hbm", "achieved": e["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": e["hbm_frac"], "traffic": None,
-                     "kernel": "stream_kernel" if vname.startswith(("stream", "wide"))
-                               else "local_sum_kernel",
+                     "kernel": {"ring": "ring_kernel", "stream": "stream_kernel",
+                                "wide": "stream_kernel"}.get(vname.split(":")[0],
+                                                             "local_sum_kernel"),
                      "variant": vname,
                      "bytes_per_launch": 8 * NPIX,
                      "batch_ms": [round(s, 5) for s in samples]},

@@ -106,8 +106,9 @@ class TuneEntry(ctypes.Structure):
                 ("param", c_int32), ("ms", c_float)]
 
 
-VAR_AUTO, VAR_TILE, VAR_STREAM, VAR_GENERIC, VAR_WIDE = range(5)
-VARIANT_NAMES = {VAR_TILE: "tile", VAR_STREAM: "stream", VAR_GENERIC: "generic", VAR_WIDE: "wide"}
+VAR_AUTO, VAR_TILE, VAR_STREAM, VAR_GENERIC, VAR_WIDE, VAR_RING = range(6)
+VARIANT_NAMES = {VAR_TILE: "tile", VAR_STREAM: "stream", VAR_GENERIC: "generic", VAR_WIDE: "wide",
+                 VAR_RING: "ring"}
 
 # Every exported entry point with its ctypes signature; tests check that the
 # library exports exactly these names.

@@ -433,7 +433,8 @@ static int tune_candidates(RunFn run, const TuneKey& key, int iters, hipStream_t
       {kVarStream, 8},      {kVarStream, 16},     {kVarStream, 24},     {kVarStream, 32},
       {kVarStream, 64},     {kVarStreamWide, 8},  {kVarStreamWide, 16}, {kVarStreamWide, 24},
       {kVarStreamWide, 32}, {kVarStreamWide, 64}, {kVarTile, 2},        {kVarTile, 4},
-      {kVarTile, 8},        {kVarGeneric, 0}};
+      {kVarTile, 8},        {kVarRing, 0},        {kVarRing, 32},       {kVarRing, 128},
+      {kVarGeneric, 0}};
   if (iters < 1) iters = 1;
   hipEvent_t e0, e1;
   hipError_t e = hipEventCreate(&e0);
@@ -495,7 +496,7 @@ int hipacc_hip_local(const hipacc_local_desc* d, void* stream) {
 }
 
 int hipacc_hip_local_variant(const hipacc_local_desc* d, int32_t variant, int32_t param, void* stream) {
-  if (variant < kVarAuto || variant > kVarStreamWide)
+  if (variant < kVarAuto || variant > kVarRing)
     return set_error(HIPACC_ERR_INVALID_ARG, "local_variant: variant %d", variant);
   return run_local(d, variant, param, static_cast<hipStream_t>(stream));
 }
@@ -506,7 +507,7 @@ int hipacc_hip_separable(const hipacc_separable_desc* d, void* stream) {
 
 int hipacc_hip_separable_variant(const hipacc_separable_desc* d, int32_t variant, int32_t param,
                                  void* stream) {
-  if (variant < kVarAuto || variant > kVarStreamWide)
+  if (variant < kVarAuto || variant > kVarRing)
     return set_error(HIPACC_ERR_INVALID_ARG, "separable_variant: variant %d", variant);
   return run_separable(d, variant, param, static_cast<hipStream_t>(stream));
 }
@@ -547,7 +548,7 @@ int hipacc_hip_tune_separable(const hipacc_separable_desc* d, int32_t iters, hip
 
 int hipacc_hip_tune_set(const hipacc_tune_entry* e) {
   if (!e) return set_error(HIPACC_ERR_INVALID_ARG, "tune_set: NULL entry");
-  if (e->variant < kVarTile || e->variant > kVarStreamWide)
+  if (e->variant < kVarTile || e->variant > kVarRing)
     return set_error(HIPACC_ERR_INVALID_ARG, "tune_set: variant %d", e->variant);
   TuneKey k{e->kind, e->kx, e->ky, e->in_dtype, e->out_dtype, e->acc_dtype == HIPACC_F32,
             e->boundary, e->fp_contract, e->weights_dev};

@@ -51,7 +51,10 @@ struct LocalArgs {
 //  stream  row-streaming strips, `param` rows per segment (hh_stream.hpp)
 //  generic one pixel per thread, any shape / mode (local.hip)
 //  wide    stream with 8 pixels per lane instead of 4
-enum Variant { kVarAuto = 0, kVarTile = 1, kVarStream = 2, kVarGeneric = 3, kVarStreamWide = 4 };
+//  ring    block row ring in LDS, `param` rows per block segment (hh_ring.hpp)
+enum Variant {
+  kVarAuto = 0, kVarTile = 1, kVarStream = 2, kVarGeneric = 3, kVarStreamWide = 4, kVarRing = 5
+};
 
 // Operator kinds the tuner keys on.
 enum TuneKind { kTuneLocal = 0, kTuneSeparable = 1, kTuneBox = 2 };
@@ -89,6 +92,7 @@ struct SepArgs {
 int launch_separable(const SepArgs& a, int acc_is_float, int fp_contract,
                      hipStream_t stream, int variant, int param);
 int dispatch_stream_local(const LocalArgs& a, int acc_is_float, int seg, int wide, hipStream_t s);
+int dispatch_ring_local(const LocalArgs& a, int seg_rows, hipStream_t s);
 int dispatch_stream_sep(const SepArgs& a, int acc_is_float, int fma, int seg, int wide,
                         hipStream_t s);
 

@@ -105,6 +105,8 @@ static int launch_local_variant(const LocalArgs& f, int acc_is_float, hipStream_
       return dispatch_stream_local(f, acc_is_float, param, 0, s);
     case kVarStreamWide:
       return dispatch_stream_local(f, acc_is_float, param, 1, s);
+    case kVarRing:
+      return acc_is_float ? dispatch_ring_local(f, param, s) : HIPACC_ERR_UNSUPPORTED;
     case kVarTile: {
       if (!acc_is_float) return dispatch_sum_int(f, param, s);
       const int st = dispatch_sum_f32(f, param, s);
@@ -115,7 +117,8 @@ static int launch_local_variant(const LocalArgs& f, int acc_is_float, hipStream_
   }
 }
 
-// variant kVarAuto: the tuned choice if any, else stream -> tile -> generic.
+// variant kVarAuto: the tuned choice if any, else ring -> stream -> tile ->
+// generic.
 // A forced variant returns HIPACC_ERR_UNSUPPORTED when it has no kernel for
 // the operator's shape.
 int launch_local(const LocalArgs& a, int acc_is_float, hipStream_t s, int variant, int param) {
@@ -150,7 +153,8 @@ int launch_local(const LocalArgs& a, int acc_is_float, hipStream_t s, int varian
     const int st = launch_local_variant(f, acc_is_float, s, v.variant, v.param);
     if (st != HIPACC_ERR_UNSUPPORTED) return st;
   }
-  int st = launch_local_variant(f, acc_is_float, s, kVarStream, 0);
+  int st = launch_local_variant(f, acc_is_float, s, kVarRing, 0);
+  if (st == HIPACC_ERR_UNSUPPORTED) st = launch_local_variant(f, acc_is_float, s, kVarStream, 0);
   if (st == HIPACC_ERR_UNSUPPORTED) st = launch_local_variant(f, acc_is_float, s, kVarTile, 0);
   if (st == HIPACC_ERR_UNSUPPORTED) st = launch_generic(a, acc_is_float, s);
   return st;

@@ -1,0 +1,122 @@
+// ring_2d.hip -- instantiations and launch geometry of the block row ring
+// (hh_ring.hpp): f32 -> f32 weighted SUM, square masks 3x3 .. 9x9.
+#include <cmath>
+
+#include "hh_ring.hpp"
+
+namespace hh {
+
+constexpr int kRingChunk = 8;   // rows per LDS chunk
+constexpr int kRingSets = 2;    // register sets of staged rows in flight
+
+// Blocks resident on the whole device at once, at most 4 per CU: on a CU
+// the oldest block is served first, so each further block lengthens the
+// tail in which the youngest finishes alone.
+template <int K, bool FMA, bool WDEV>
+static int ring_slots() {
+  static const int slots = [] {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu,
+            reinterpret_cast<const void*>(&ring_kernel<K, FMA, WDEV, kRingChunk, kRingSets>), 256,
+            0) != hipSuccess)
+      return 0;
+    return cus * (per_cu < 4 ? per_cu : 4);
+  }();
+  return slots;
+}
+
+// Time of a row of an edge strip (per-pixel loads and stores) relative to a
+// row of an interior strip, measured at 4096^2: the VALU work of the larger
+// masks hides most of the extra memory instructions.
+static float ring_edge_cost(int k) { return k <= 5 ? 1.55f : 1.15f; }
+
+// Classifies the strips the way ring_kernel does (edge: the strip's staged
+// columns leave the raw-readable region, or its stored columns are not all
+// full vectors) and sizes the segments.  seg_rows > 0 forces the interior
+// segment height; 0 picks it so that one round of items fills `slots`.
+static RingGeom ring_geom(const LocalArgs& a, int k, int seg_rows, int slots) {
+  const Geom& g = a.g;
+  RingGeom rg;
+  rg.strips = ceil_div(g.is_w, kRingStripW);
+  int lx = g.lo_x, hx = g.hi_x;
+  if (g.boundary == HIPACC_BOUNDARY_UNDEFINED) {
+    lx = 0;
+    hx = g.in_imgw;
+  }
+  auto edge = [&](int strip) {
+    const int xs = strip * kRingStripW - kRingV, ix0 = g.in_ox + xs;
+    return !(a.in_vec_ok && a.out_vec_ok && ix0 >= lx && ix0 + 64 * kRingV <= hx &&
+             xs + 63 * kRingV <= g.is_w);
+  };
+  rg.nl = 0;
+  while (rg.nl < rg.strips && edge(rg.nl)) ++rg.nl;
+  rg.nr = 0;
+  while (rg.nl + rg.nr < rg.strips && edge(rg.strips - 1 - rg.nr)) ++rg.nr;
+  const int ne = rg.nl + rg.nr, nm = rg.strips - ne;
+  const float ce = (nm > 0 && ne > 0) ? ring_edge_cost(k) : 1.0f;
+  auto size = [&](int rows) {
+    int re = (int)((float)rows / ce);
+    if (re < 8) re = 8;
+    rg.seg_rows = rows < g.is_h ? rows : g.is_h;
+    rg.seg_rows_e = re < g.is_h ? re : g.is_h;
+    rg.segs = ceil_div(g.is_h, rg.seg_rows);
+    rg.segs_e = ceil_div(g.is_h, rg.seg_rows_e);
+    rg.per_img = nm * rg.segs + ne * rg.segs_e;
+    rg.items = rg.per_img * g.batch;
+  };
+  if (seg_rows > 0) {
+    size(seg_rows);
+  } else {
+    // interior-row units of work, spread over the slots; then taller
+    // segments until the rounding leaves no item for a second round
+    const double units = ((double)nm + ne * ce) * g.is_h * g.batch;
+    int rows = slots > 0 ? (int)std::ceil(units / slots) : g.is_h;
+    if (rows < 16) rows = 16;
+    size(rows);
+    for (int it = 0; it < 64 && slots > 0 && rg.items > slots && rows < g.is_h; ++it) {
+      rows += rows / 32 + 1;
+      size(rows);
+    }
+  }
+  return rg;
+}
+
+template <int K, bool FMA, bool WDEV>
+static int launch_ring(const LocalArgs& a0, int seg_rows, hipStream_t s) {
+  LocalArgs a = a0;
+  stream_align(a.g, kRingV, a.in_vec_ok, a.out_vec_ok);
+  const long long strips = ceil_div(a.g.is_w, kRingStripW);
+  if (strips * a.g.is_h * a.g.batch > (1ll << 31))
+    return set_error(HIPACC_ERR_UNSUPPORTED, "ring kernel: %lld strip rows",
+                     strips * a.g.is_h * a.g.batch);
+  const RingGeom rg = ring_geom(a, K, seg_rows, ring_slots<K, FMA, WDEV>());
+  hipLaunchKernelGGL((ring_kernel<K, FMA, WDEV, kRingChunk, kRingSets>), dim3(rg.items), dim3(256),
+                     0, s, a, rg);
+  return hip_status(hipGetLastError(), "ring_kernel launch");
+}
+
+int dispatch_ring_local(const LocalArgs& a, int seg_rows, hipStream_t s) {
+  const Geom& g = a.g;
+  if (a.reduce != HIPACC_REDUCE_SUM || a.mask_w != a.mask_h || g.in_dtype != HIPACC_F32 ||
+      g.out_dtype != HIPACC_F32 || g.post_div != 0.0f || g.cvt == HIPACC_CVT_ROUND)
+    return HIPACC_ERR_UNSUPPORTED;
+  // byte offsets of the buffer accesses are 32-bit
+  if ((long long)g.in_imgw * 4 >= (1ll << 31) || (long long)g.is_w * 4 >= (1ll << 31))
+    return HIPACC_ERR_UNSUPPORTED;
+  const bool wd = a.wdev != nullptr;
+#define HH_RING(K)                                                                     \
+  if (a.mask_w == K) {                                                                 \
+    if (a.fma) return wd ? launch_ring<K, true, true>(a, seg_rows, s)                  \
+                         : launch_ring<K, true, false>(a, seg_rows, s);                \
+    return wd ? launch_ring<K, false, true>(a, seg_rows, s)                            \
+              : launch_ring<K, false, false>(a, seg_rows, s);                          \
+  }
+  HH_RING(3) HH_RING(5) HH_RING(7) HH_RING(9)
+#undef HH_RING
+  return HIPACC_ERR_UNSUPPORTED;
+}
+
+}  // namespace hh

@@ -59,7 +59,7 @@ int tune_load(const char* path) {
     if (std::sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %f", &k.kind, &k.kx, &k.ky, &k.in_dt,
                     &k.out_dt, &k.acc_f, &k.boundary, &k.fma, &k.wdev, &v.variant, &v.param,
                     &v.ms) == 12) {
-      if (v.variant < kVarTile || v.variant > kVarStreamWide) continue;
+      if (v.variant < kVarTile || v.variant > kVarRing) continue;
       tune_store(k, v);
       ++n;
     }

@@ -410,15 +410,18 @@ int hipacc_hip_pyramid_run_direct(hipacc_pyramid_plan* plan, void* stream);
 int hipacc_hip_pyramid_plan_destroy(hipacc_pyramid_plan* plan);
 
 /* ---- kernel variants and the autotuner ------------------------------- */
-/* A local or separable operator runs one of three kernel variants:
+/* A local or separable operator runs one of these kernel variants:
  *   1 tile    LDS-staged tile + halo; param = output rows per wave (2/4/8)
  *   2 stream  row-streaming strips with a register prefetch ring and
  *             cross-lane halos; param = output rows per wave segment
  *   3 generic one pixel per thread, any shape and reduction mode
  *   4 wide    stream with 8 pixels per lane instead of 4
+ *   5 ring    (local f32 -> f32 weighted SUM, square masks 3..9) block row
+ *             ring in LDS; param = output rows per block segment, 0 = sized
+ *             so that every block is resident at once
  * All variants compute identical results.  The plain entry points use the
  * tuned variant of the operator's shape when the tune cache has one, else
- * stream, else tile, else generic.  The tuner times every candidate on the
+ * ring, else stream, else tile, else generic.  The tuner times every candidate on the
  * caller's buffers (the output is overwritten) and records the fastest per
  * (kind, mask size, pixel types, accumulator, boundary, contraction, device
  * weights).  The cache lives in the process and can be saved / loaded.
@@ -429,7 +432,7 @@ typedef struct hipacc_tune_entry {
   int32_t kind;        /* 0 local weighted SUM, 1 separable, 2 box sum         */
   int32_t kx, ky;
   int32_t in_dtype, out_dtype, acc_dtype, boundary, fp_contract, weights_dev;
-  int32_t variant;     /* 1 tile, 2 stream, 3 generic, 4 wide stream          */
+  int32_t variant;     /* 1 tile, 2 stream, 3 generic, 4 wide stream, 5 ring  */
   int32_t param;
   float   ms;          /* measured time per launch                            */
 } hipacc_tune_entry;

@@ -1,5 +1,6 @@
 """Every kernel variant the autotuner can pick (stream with several segment
-heights, tile with several rows per wave, generic) produces the oracle's
+heights, tile with several rows per wave, ring with several block segment
+heights, generic) produces the oracle's
 result bit for bit, so the tuned choice never changes a result."""
 import numpy as np
 import pytest
@@ -14,7 +15,8 @@ pytestmark = pytest.mark.gpu
 
 VARIANTS = [(capi.VAR_STREAM, 0), (capi.VAR_STREAM, 8), (capi.VAR_STREAM, 13),
             (capi.VAR_STREAM, 64), (capi.VAR_WIDE, 0), (capi.VAR_WIDE, 7), (capi.VAR_WIDE, 32),
-            (capi.VAR_TILE, 2), (capi.VAR_TILE, 4), (capi.VAR_TILE, 8), (capi.VAR_GENERIC, 0)]
+            (capi.VAR_TILE, 2), (capi.VAR_TILE, 4), (capi.VAR_TILE, 8), (capi.VAR_GENERIC, 0),
+            (capi.VAR_RING, 0), (capi.VAR_RING, 5), (capi.VAR_RING, 16), (capi.VAR_RING, 37)]
 MODES = [capi.UNDEFINED, capi.CLAMP, capi.REPEAT, capi.MIRROR, capi.CONSTANT]
 
 
