@@ -69,6 +69,26 @@ template <int QR> struct RingStage<true, QR> {
   int rc;
 };
 
+// Waits until at most N vector memory operations of this wave are
+// outstanding (they complete in issue order), and a workgroup barrier; both
+// are compiler barriers for memory operations.  The barrier carries no wait
+// of its own, so direct loads stay in flight across it.
+template <int N> __device__ __forceinline__ void ring_wait_vm() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+// ring_wait_vm<N> for a count known only after unrolling (waits for at most
+// min(n, N) outstanding operations).
+template <int N> __device__ __forceinline__ void ring_wait_vm_le(int n) {
+  if constexpr (N == 0) {
+    ring_wait_vm<0>();
+  } else {
+    if (n >= N) ring_wait_vm<N>();
+    else ring_wait_vm_le<N - 1>(n);
+  }
+}
+__device__ __forceinline__ void ring_barrier() { asm volatile("s_barrier" ::: "memory"); }
+
 __device__ __forceinline__ float4 ring_merge(const float4& a, const float4& b) {
   // one of the two is all zero bits: OR keeps the other bit for bit
   return make_float4(__builtin_bit_cast(float, __builtin_bit_cast(unsigned, a.x) | __builtin_bit_cast(unsigned, b.x)),
@@ -82,9 +102,11 @@ __device__ __forceinline__ float4 ring_merge(const float4& a, const float4& b) {
 // and stores where needed); otherwise one 16-byte load and store per lane
 // and row.
 // The block computes output rows [y0, y0 + nout) of its strip; xl is the
-// iteration-space column of the lane's first pixel.  CH rows per LDS chunk,
-// NS register sets of staged rows in flight.
-template <int K, bool FMA, bool WDEV, int CH, int NS, bool EDGE>
+// iteration-space column of the lane's first pixel.  CH rows per LDS chunk.
+// Interior strips stage their rows with direct global->LDS loads, ND chunks
+// ahead, through a ring of ND+2 chunks; edge strips stage through NS
+// register sets and a ring of three chunks.
+template <int K, bool FMA, bool WDEV, int CH, int NS, int ND, bool EDGE>
 __device__ __forceinline__ void ring_body(const LocalArgs& a, float4* __restrict__ lds,
                                           const float* __restrict__ in, float* __restrict__ out,
                                           int xl, int y0, int nout) {
@@ -96,6 +118,7 @@ __device__ __forceinline__ void ring_body(const LocalArgs& a, float4* __restrict
   // the larger masks, whose VALU work hides the per-pixel loads, keep the
   // registers and load every pixel of an edge strip separately.
   constexpr bool SPLIT = EDGE && K <= 5;
+  constexpr int NSLOT = EDGE ? 3 : ND + 2;   // chunks in the LDS ring
   const Geom& g = a.g;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) & 3);
@@ -158,7 +181,7 @@ __device__ __forceinline__ void ring_body(const LocalArgs& a, float4* __restrict
   };
   // CONSTANT pixels are selected here, after the loads have landed.
   auto write_chunk = [&](int c, const RingStage<SPLIT, QR>& st) {
-    const int slot = c % 3;
+    const int slot = c % NSLOT;
 #pragma unroll
     for (int i = 0; i < QR; ++i) {
       float4 v;
@@ -202,7 +225,7 @@ __device__ __forceinline__ void ring_body(const LocalArgs& a, float4* __restrict
     for (int t = 0; t < NT; ++t) {
       const int off = jw + t;              // row of chunk s (or of chunk s+1 past CH)
       const int c = off >= CH ? 1 : 0;
-      const int slot = (s + c) % 3;
+      const int slot = (s + c) % NSLOT;
       const float4 q = lds[(slot * CH + off - c * CH) * 64 + lane];
       const float cur[V] = {q.x, q.y, q.z, q.w};
       constexpr int NWIN = V + 2 * R;
@@ -247,37 +270,77 @@ __device__ __forceinline__ void ring_body(const LocalArgs& a, float4* __restrict
     }
   };
 
-  // Chunk c lives in LDS slot c % 3.  Step s computes output chunk s from
-  // chunks s and s+1, after writing chunk s+2 (loaded NS steps earlier) and
-  // issuing the loads of chunk s+2+NS.
   const int nos = (nout + CH - 1) / CH;
-  RingStage<SPLIT, QR> sets[NS];
-  {
-    const RingStage<SPLIT, QR> c0 = load_chunk(0), c1 = load_chunk(1);
-    write_chunk(0, c0);
-    write_chunk(1, c1);
-  }
+  if constexpr (!EDGE) {
+    // Direct loads: chunk c lands in slot c % NSLOT.  Step s waits for its
+    // own loads of chunk s+1 (the counter also holds the loads and stores
+    // issued after them), meets the other waves at the barrier, issues the
+    // loads of chunk s+ND+1 into the slot chunk s-1 held, and computes
+    // output chunk s from chunks s and s+1.
+    auto issue = [&](int c) {
+      const int slot = c % NSLOT;
 #pragma unroll
-  for (int u = 0; u < NS; ++u) sets[(u + 2) % NS] = load_chunk(u + 2);
-  __syncthreads();
-  int s = 0;
-  for (; s + NS <= nos; s += NS) {
+      for (int i = 0; i < QR; ++i) {
+        const int k = min(c * CH + wave + 4 * i, nin - 1);
+        bool rc;
+        const int y = row_src(g, iy0 + k, rc);
+        __builtin_amdgcn_global_load_lds(
+            static_cast<const void*>(in + (long long)y * g.in_stride + ix),
+            (__attribute__((address_space(3))) void*)(lds + (slot * CH + wave + 4 * i) * 64), 16,
+            0, 0);
+      }
+    };
+#pragma unroll
+    for (int c = 0; c <= ND; ++c) issue(c);
+    // first ND steps: chunk s+1 came from the prologue
+#pragma unroll
+    for (int s = 0; s < ND; ++s) {
+      if (s < nos) {
+        ring_wait_vm_le<QR * (2 * ND - 1)>(QR * (ND - 1 + s));
+        ring_barrier();
+        issue(s + ND + 1);
+        compute(s);
+      }
+    }
+    for (int s = ND; s < nos; ++s) {
+      ring_wait_vm<2 * QR * (ND - 1)>();
+      ring_barrier();
+      issue(s + ND + 1);
+      compute(s);
+    }
+    ring_wait_vm<0>();                   // no load may land after the block ends
+  } else {
+    // Register sets: chunk c lives in LDS slot c % 3.  Step s computes output
+    // chunk s from chunks s and s+1, after writing chunk s+2 (loaded NS steps
+    // earlier) and issuing the loads of chunk s+2+NS.
+    RingStage<SPLIT, QR> sets[NS];
+    {
+      const RingStage<SPLIT, QR> c0 = load_chunk(0), c1 = load_chunk(1);
+      write_chunk(0, c0);
+      write_chunk(1, c1);
+    }
+#pragma unroll
+    for (int u = 0; u < NS; ++u) sets[(u + 2) % NS] = load_chunk(u + 2);
+    __syncthreads();
+    int s = 0;
+    for (; s + NS <= nos; s += NS) {
+#pragma unroll
+      for (int u = 0; u < NS; ++u) {
+        const int k = (u + 2) % NS;
+        write_chunk(s + u + 2, sets[k]);
+        sets[k] = load_chunk(s + u + 2 + NS);
+        compute(s + u);
+        __syncthreads();
+      }
+    }
 #pragma unroll
     for (int u = 0; u < NS; ++u) {
-      const int k = (u + 2) % NS;
-      write_chunk(s + u + 2, sets[k]);
-      sets[k] = load_chunk(s + u + 2 + NS);
-      compute(s + u);
-      __syncthreads();
-    }
-  }
-#pragma unroll
-  for (int u = 0; u < NS; ++u) {
-    if (s + u < nos) {
-      const int k = (u + 2) % NS;
-      write_chunk(s + u + 2, sets[k]);
-      compute(s + u);
-      __syncthreads();
+      if (s + u < nos) {
+        const int k = (u + 2) % NS;
+        write_chunk(s + u + 2, sets[k]);
+        compute(s + u);
+        __syncthreads();
+      }
     }
   }
 }
@@ -314,9 +377,9 @@ __device__ __forceinline__ RingItem ring_item(const RingGeom& rg, const Geom& g,
 }
 
 // One item per block of 4 waves.
-template <int K, bool FMA, bool WDEV, int CH, int NS>
+template <int K, bool FMA, bool WDEV, int CH, int NS, int ND>
 __global__ __launch_bounds__(256) void ring_kernel(const LocalArgs a, const RingGeom rg) {
-  __shared__ __align__(16) float4 lds[3 * CH * 64];
+  __shared__ __align__(16) float4 lds[(ND + 2 > 3 ? ND + 2 : 3) * CH * 64];
   const Geom& g = a.g;
   if ((int)blockIdx.x >= rg.items) return;
   const RingItem it = ring_item(rg, g, blockIdx.x);
@@ -327,12 +390,19 @@ __global__ __launch_bounds__(256) void ring_kernel(const LocalArgs a, const Ring
   int lx, hx, ly, hy;
   raw_region(g, lx, hx, ly, hy);
   const int ix0 = g.in_ox + xs;
-  const bool cols_raw = a.in_vec_ok && a.out_vec_ok && ix0 >= lx && ix0 + 64 * kRingV <= hx &&
-                        xs + 63 * kRingV <= g.is_w;
-  if (cols_raw)
-    ring_body<K, FMA, WDEV, CH, NS, false>(a, lds, in, out, xs + lane * kRingV, it.y0, it.nout);
+  // interior: raw columns, full vector stores, and (CONSTANT) no row outside
+  // the window -- the direct loads cannot substitute the constant
+  const int iy0 = g.in_oy + it.y0 - K / 2;
+  const bool raw = a.in_vec_ok && a.out_vec_ok && ix0 >= lx && ix0 + 64 * kRingV <= hx &&
+                   xs + 63 * kRingV <= g.is_w &&
+                   (g.boundary != HIPACC_BOUNDARY_CONSTANT ||
+                    (iy0 >= g.lo_y && iy0 + it.nout + K - 1 <= g.hi_y));
+  if (raw)
+    ring_body<K, FMA, WDEV, CH, NS, ND, false>(a, lds, in, out, xs + lane * kRingV, it.y0,
+                                               it.nout);
   else
-    ring_body<K, FMA, WDEV, CH, NS, true>(a, lds, in, out, xs + lane * kRingV, it.y0, it.nout);
+    ring_body<K, FMA, WDEV, CH, NS, ND, true>(a, lds, in, out, xs + lane * kRingV, it.y0,
+                                              it.nout);
 }
 
 }  // namespace hh

@@ -7,7 +7,8 @@
 namespace hh {
 
 constexpr int kRingChunk = 8;   // rows per LDS chunk
-constexpr int kRingSets = 2;    // register sets of staged rows in flight
+constexpr int kRingSets = 2;    // register sets of staged rows in flight (edge strips)
+constexpr int kRingDepth = 1;   // chunks of direct loads in flight (interior strips)
 
 // Blocks resident on the whole device at once, at most 4 per CU: on a CU
 // the oldest block is served first, so each further block lengthens the
@@ -20,7 +21,7 @@ static int ring_slots() {
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu,
-            reinterpret_cast<const void*>(&ring_kernel<K, FMA, WDEV, kRingChunk, kRingSets>), 256,
+            reinterpret_cast<const void*>(&ring_kernel<K, FMA, WDEV, kRingChunk, kRingSets, kRingDepth>), 256,
             0) != hipSuccess)
       return 0;
     return cus * (per_cu < 4 ? per_cu : 4);
@@ -93,7 +94,7 @@ static int launch_ring(const LocalArgs& a0, int seg_rows, hipStream_t s) {
     return set_error(HIPACC_ERR_UNSUPPORTED, "ring kernel: %lld strip rows",
                      strips * a.g.is_h * a.g.batch);
   const RingGeom rg = ring_geom(a, K, seg_rows, ring_slots<K, FMA, WDEV>());
-  hipLaunchKernelGGL((ring_kernel<K, FMA, WDEV, kRingChunk, kRingSets>), dim3(rg.items), dim3(256),
+  hipLaunchKernelGGL((ring_kernel<K, FMA, WDEV, kRingChunk, kRingSets, kRingDepth>), dim3(rg.items), dim3(256),
                      0, s, a, rg);
   return hip_status(hipGetLastError(), "ring_kernel launch");
 }
