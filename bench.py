@@ -66,7 +66,7 @@ def time_calls(calls, steps, warmup):
     return e0.elapsed_time(e1) / steps
 
 
-HEAD_CANDIDATES = ("ring:0", "ring:48", "stream:12", "stream:16", "wide:16", "tile:4")
+HEAD_CANDIDATES = ("ring:0", "stream:12", "stream:16", "wide:16", "tile:4")
 
 
 def parse_variant(text):
