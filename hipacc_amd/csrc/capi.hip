@@ -645,7 +645,6 @@ int hipacc_hip_reduce(const hipacc_view* img, int32_t reduce, int32_t acc_dtype,
   a.vec_ok = rows_aligned(*img, 16);
   a.result = result_dev;
   a.partials = static_cast<unsigned long long*>(workspace);
-  a.ticket = reinterpret_cast<unsigned int*>(static_cast<char*>(workspace) + 1024 * sizeof(unsigned long long));
   hipStream_t s = static_cast<hipStream_t>(stream);
   Timer tm(s);
   return tm.finish(launch_reduce(a, img->dtype, acc_dtype, s));

@@ -120,7 +120,6 @@ struct ReduceArgs {
   int vec_ok;
   void* result;
   unsigned long long* partials;   // kRedBlocks entries
-  unsigned int* ticket;
 };
 int launch_reduce(const ReduceArgs& a, int dt, int acc_dt, hipStream_t s);
 

@@ -3,11 +3,12 @@
 //
 // Reduction: a fixed grid of blocks walks the window's rows (block b takes
 // rows b, b+G, ...; threads stride along the row with 16-byte loads when the
-// row is aligned).  Each wave folds its 64 lanes with cross-lane shuffles,
+// row is aligned, keeping the loads of 4 rows in flight together).  Each wave folds its 64 lanes with cross-lane shuffles,
 // one LDS level folds the 4 waves, and block partials go to the workspace.
-// The block that takes the last ticket folds the partials in a fixed order,
-// writes the result and re-arms the ticket, so one launch does the whole
-// reduction and the result does not depend on scheduling.
+// A second one-block kernel folds the partials in a fixed order and writes
+// the result, so the result does not depend on scheduling.  (A last-block
+// ticket would need a device-scope release per block, which writes back
+// the L2 of each XCD and costs more than the second launch.)
 #include "hh_tile.hpp"
 
 namespace hh {
@@ -65,27 +66,81 @@ template <typename A> __device__ __forceinline__ A block_fold(int mode, A v, A* 
   return v;
 }
 
-template <typename T, typename A>
+// MODE is a template argument: a runtime mode would put a switch, with its
+// branches, into the per-pixel loop.
+template <typename T, typename A, int MODE>
 __global__ __launch_bounds__(256) void reduce_kernel(const ReduceArgs a) {
   constexpr int VEC = 16 / (int)sizeof(T);
+  constexpr int U = 4;   // rows whose loads a thread keeps in flight together
   __shared__ A sm[4];
-  __shared__ bool last;
-  const int mode = a.mode;
+  constexpr int mode = MODE;
+  const int G = gridDim.x;
   A acc = red_identity<A>(mode);
-  for (int y = blockIdx.x; y < a.h; y += gridDim.x) {
-    const T* row = static_cast<const T*>(a.in) + (long long)(a.oy + y) * a.stride + a.ox;
-    if (a.vec_ok) {
-      const int nv = a.w / VEC;
-      for (int i = threadIdx.x; i < nv; i += 256) {
-        const uint4 w = reinterpret_cast<const uint4*>(row)[i];
+  const T* base = static_cast<const T*>(a.in) + (long long)a.oy * a.stride + a.ox;
+  if (a.vec_ok && a.stride == a.w) {
+    // rows back to back: one grid-stride sweep, so that the whole grid reads
+    // one contiguous stretch of memory at a time
+    const long long nv = (long long)a.h * a.w / VEC;
+    const long long st = (long long)G * 256;
+    const uint4* v = reinterpret_cast<const uint4*>(base);
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    for (; i + (U - 1) * st < nv; i += U * st) {
+      uint4 w[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) w[u] = v[i + u * st];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
         T t[VEC];
-        __builtin_memcpy(t, &w, 16);
+        __builtin_memcpy(t, &w[u], 16);
 #pragma unroll
         for (int k = 0; k < VEC; ++k) acc = red_op<A>(mode, acc, (A)t[k]);
       }
-      for (int x = nv * VEC + threadIdx.x; x < a.w; x += 256) acc = red_op<A>(mode, acc, (A)row[x]);
+    }
+    for (; i < nv; i += st) {
+      const uint4 w = v[i];
+      T t[VEC];
+      __builtin_memcpy(t, &w, 16);
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) acc = red_op<A>(mode, acc, (A)t[k]);
+    }
+    const long long n = (long long)a.h * a.w;
+    for (long long x = nv * VEC + (long long)blockIdx.x * 256 + threadIdx.x; x < n; x += st)
+      acc = red_op<A>(mode, acc, (A)base[x]);
+  } else
+  // block b folds rows b, b+G, ...; a thread loads the same vector of U of
+  // its block's rows at once
+  for (int y0 = blockIdx.x; y0 < a.h; y0 += U * G) {
+    if (a.vec_ok) {
+      const int nv = a.w / VEC;
+      for (int i = threadIdx.x; i < nv; i += 256) {
+        uint4 w[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int y = y0 + u * G;
+          if (y < a.h) w[u] = reinterpret_cast<const uint4*>(base + (long long)y * a.stride)[i];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (y0 + u * G >= a.h) break;
+          T t[VEC];
+          __builtin_memcpy(t, &w[u], 16);
+#pragma unroll
+          for (int k = 0; k < VEC; ++k) acc = red_op<A>(mode, acc, (A)t[k]);
+        }
+      }
+      for (int u = 0; u < U; ++u) {
+        const int y = y0 + u * G;
+        if (y >= a.h) break;
+        const T* row = base + (long long)y * a.stride;
+        for (int x = nv * VEC + threadIdx.x; x < a.w; x += 256) acc = red_op<A>(mode, acc, (A)row[x]);
+      }
     } else {
-      for (int x = threadIdx.x; x < a.w; x += 256) acc = red_op<A>(mode, acc, (A)row[x]);
+      for (int u = 0; u < U; ++u) {
+        const int y = y0 + u * G;
+        if (y >= a.h) break;
+        const T* row = base + (long long)y * a.stride;
+        for (int x = threadIdx.x; x < a.w; x += 256) acc = red_op<A>(mode, acc, (A)row[x]);
+      }
     }
   }
   acc = block_fold<A>(mode, acc, sm);
@@ -93,32 +148,42 @@ __global__ __launch_bounds__(256) void reduce_kernel(const ReduceArgs a) {
     unsigned long long bits = 0;
     __builtin_memcpy(&bits, &acc, sizeof(A));
     a.partials[blockIdx.x] = bits;
-    __threadfence();
-    const unsigned int t = atomicAdd(a.ticket, 1u);
-    last = (t == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  A v = red_identity<A>(mode);
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += 256) {
-    const unsigned long long bits = __atomic_load_n(&a.partials[i], __ATOMIC_RELAXED);
+}
+
+// Folds the n block partials in a fixed order (one block; the kernel
+// boundary orders it after every partial is written).
+template <typename A, int MODE>
+__global__ __launch_bounds__(256) void reduce_final_kernel(const unsigned long long* partials,
+                                                           int n, A* result) {
+  __shared__ A sm[4];
+  A v = red_identity<A>(MODE);
+  for (int i = threadIdx.x; i < n; i += 256) {
     A p;
-    __builtin_memcpy(&p, &bits, sizeof(A));
-    v = red_op<A>(mode, v, p);
+    __builtin_memcpy(&p, &partials[i], sizeof(A));
+    v = red_op<A>(MODE, v, p);
   }
-  __syncthreads();
-  v = block_fold<A>(mode, v, sm);
-  if (threadIdx.x == 0) {
-    *static_cast<A*>(a.result) = v;
-    *a.ticket = 0u;  // re-arm for the next launch on this workspace
-  }
+  v = block_fold<A>(MODE, v, sm);
+  if (threadIdx.x == 0) *result = v;
 }
 
 template <typename T, typename A>
 static int launch_reduce_t(const ReduceArgs& a, hipStream_t s) {
-  const int blocks = a.h < kRedBlocks ? a.h : kRedBlocks;
-  hipLaunchKernelGGL((reduce_kernel<T, A>), dim3(blocks), dim3(256), 0, s, a);
+  const long long n = (long long)a.w * a.h;
+  const int flat_blocks = (int)((n + 4095) / 4096 < kRedBlocks ? (n + 4095) / 4096 : kRedBlocks);
+  const int blocks = (a.vec_ok && a.stride == a.w) ? (flat_blocks > 0 ? flat_blocks : 1)
+                                                   : (a.h < kRedBlocks ? a.h : kRedBlocks);
+#define HH_RED(M)                                                                         \
+  hipLaunchKernelGGL((reduce_kernel<T, A, M>), dim3(blocks), dim3(256), 0, s, a);         \
+  hipLaunchKernelGGL((reduce_final_kernel<A, M>), dim3(1), dim3(256), 0, s, a.partials, blocks, \
+                     static_cast<A*>(a.result));
+  switch (a.mode) {
+    case HIPACC_REDUCE_MIN: HH_RED(HIPACC_REDUCE_MIN) break;
+    case HIPACC_REDUCE_MAX: HH_RED(HIPACC_REDUCE_MAX) break;
+    case HIPACC_REDUCE_PROD: HH_RED(HIPACC_REDUCE_PROD) break;
+    default: HH_RED(HIPACC_REDUCE_SUM) break;
+  }
+#undef HH_RED
   return hip_status(hipGetLastError(), "reduce_kernel launch");
 }
 
