@@ -73,16 +73,19 @@ __device__ __forceinline__ float pyr_read(const float* g, int stride, int w, int
 // G[l+1] values the tile's LF reads and the ones it owns (columns x0/2..,
 // rows y0/2..) are computed -- the 5x5 convolution evaluated only at the NN
 // sample points -- the owned ones are stored, then L[l] = G[l] - LF(G[l+1]).
-template <int TH, bool FMA>
+// SNUM / 4 bounds the NN step s = W / W1 of every image in the work list:
+// 12 (s <= 3, any image) or 9 (s <= 2.25: images at least 9 pixels wide and
+// high at this level), which sizes the staged region.
+template <int TW, int TH, int SNUM, bool FMA>
 __global__ __launch_bounds__(256) void pyramid_level_kernel(const PyrLevelArgs a) {
-  constexpr int TW = 64;
   constexpr int UW = TW / 2 + 4, UH = TH / 2 + 4;   // G[l+1] region capacity
-  // G[l] region capacity: the NN map has step s = W/W1 <= 3, so the blur
-  // taps of UW x UH sample points span at most 3*(UW-1)+5 x 3*(UH-1)+5.
-  constexpr int RW = 3 * (UW - 1) + 5, RH = 3 * (UH - 1) + 5;
+  // G[l] region capacity: the blur taps of UW x UH sample points, s apart,
+  // span at most ceil(s*(UW-1)) + 5 x ceil(s*(UH-1)) + 5.
+  constexpr int RW = (SNUM * (UW - 1) + 3) / 4 + 5, RH = (SNUM * (UH - 1) + 3) / 4 + 5;
   __shared__ float g1s[UH * UW];
   __shared__ float gls[RH * RW];
   const int l = a.level;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
   for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
     const PyrTile tile = a.tiles[t];
@@ -119,9 +122,30 @@ __global__ __launch_bounds__(256) void pyramid_level_kernel(const PyrLevelArgs a
     if (uw > UW || uh > UH || rw > RW || rh > RH) continue;   // cannot happen (see above)
 
     __syncthreads();  // previous tile's LDS reads are done
-    for (int i = threadIdx.x; i < rw * rh; i += 256) {
-      const int r = i / rw, c = i - r * rw;
-      gls[r * RW + c] = pyr_read(G, gs, W, H, a.boundary, rx0 + c, ry0 + r);
+    // wave w stages rows w, w+4, ...; lane c the columns c, c+64, ...  Inside
+    // the image every load of the region is issued before the first LDS
+    // write, so the whole region is in flight at once.
+    if (rx0 >= 0 && ry0 >= 0 && rx1 < W && ry1 < H) {
+      constexpr int NR = (RH + 3) / 4, NC = (RW + 63) / 64;
+      float v[NR][NC];
+#pragma unroll
+      for (int k = 0; k < NR; ++k) {
+        const int r = wave + 4 * k;
+        const float* src = G + (long long)(ry0 + min(r, rh - 1)) * gs + rx0;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[k][c] = src[min(lane + 64 * c, rw - 1)];
+      }
+#pragma unroll
+      for (int k = 0; k < NR; ++k) {
+        const int r = wave + 4 * k;
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          if (r < rh && lane + 64 * c < rw) gls[r * RW + lane + 64 * c] = v[k][c];
+      }
+    } else {
+      for (int r = wave; r < rh; r += 4)
+        for (int c = lane; c < rw; c += 64)
+          gls[r * RW + c] = pyr_read(G, gs, W, H, a.boundary, rx0 + c, ry0 + r);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < uw * uh; i += 256) {
@@ -147,24 +171,27 @@ __global__ __launch_bounds__(256) void pyramid_level_kernel(const PyrLevelArgs a
     }
     __syncthreads();
 
-    // a lane keeps one column (its LF column weights are computed once) and
-    // walks the rows tid/64, +4, ...
-    const int x = x0 + (threadIdx.x & 63);
-    if (x < W) {
-      int xi;
-      float fx;
-      interp_lf(interp_map(su, x), xi, fx);
-      const int xa = xi - ux0, xb = min(xi + 1, W1 - 1) - ux0;
-      for (int r = threadIdx.x >> 6; r < TH; r += 4) {
-        const int y = y0 + r;
-        if (y >= H) break;
-        int yi;
-        float fy;
-        interp_lf(interp_map(sv, y), yi, fy);
-        const int ya = yi - uy0, yb = min(yi + 1, H1 - 1) - uy0;
-        const float lf = interp_bilinear(fx, fy, g1s[ya * UW + xa], g1s[ya * UW + xb],
-                                         g1s[yb * UW + xa], g1s[yb * UW + xb]);
-        L[(long long)y * ls + x] = gls[(y - ry0) * RW + (x - rx0)] - lf;
+    // a lane keeps the columns lane, lane+64, ... (their LF column weights
+    // are computed once) and walks the rows wave, wave+4, ...
+#pragma unroll
+    for (int cx = 0; cx < TW / 64; ++cx) {
+      const int x = x0 + lane + 64 * cx;
+      if (x < W) {
+        int xi;
+        float fx;
+        interp_lf(interp_map(su, x), xi, fx);
+        const int xa = xi - ux0, xb = min(xi + 1, W1 - 1) - ux0;
+        for (int r = wave; r < TH; r += 4) {
+          const int y = y0 + r;
+          if (y >= H) break;
+          int yi;
+          float fy;
+          interp_lf(interp_map(sv, y), yi, fy);
+          const int ya = yi - uy0, yb = min(yi + 1, H1 - 1) - uy0;
+          const float lf = interp_bilinear(fx, fy, g1s[ya * UW + xa], g1s[ya * UW + xb],
+                                           g1s[yb * UW + xa], g1s[yb * UW + xb]);
+          L[(long long)y * ls + x] = gls[(y - ry0) * RW + (x - rx0)] - lf;
+        }
       }
     }
   }
@@ -180,44 +207,49 @@ struct hipacc_pyramid_plan {
   float w[25];
   hh::PyrImage* d_imgs = nullptr;
   hh::PyrTile* d_tiles = nullptr;
-  std::vector<int> off, cnt;
+  std::vector<int> off, cnt;     // narrow tiles (64 x tile_rows) per level
+  std::vector<int> offw, cntw;   // wide tiles (128 x 32) per level
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
 
 namespace hh {
 
-template <int TH, bool FMA>
-static int launch_level_t(const PyrLevelArgs& a, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL((pyramid_level_kernel<TH, FMA>), dim3(nblocks), dim3(256), 0, s, a);
+constexpr int kPyrWideMin = 512;    // wide tiles of a level worth a wide launch
+
+template <int TW, int TH, int SNUM, bool FMA>
+static int launch_level_t(const PyrLevelArgs& a, hipStream_t s) {
+  // one block per tile: the hardware keeps several tiles per CU in flight,
+  // hiding the staging latency of one behind the arithmetic of another
+  hipLaunchKernelGGL((pyramid_level_kernel<TW, TH, SNUM, FMA>), dim3(a.ntiles), dim3(256), 0, s, a);
   return hip_status(hipGetLastError(), "pyramid_level_kernel launch");
+}
+
+template <bool FMA>
+static int launch_level(const PyrLevelArgs& a, bool wide, int tile_rows, hipStream_t s) {
+  if (wide) return launch_level_t<128, 32, 9, FMA>(a, s);
+  return tile_rows == 8    ? launch_level_t<64, 8, 12, FMA>(a, s)
+         : tile_rows == 32 ? launch_level_t<64, 32, 12, FMA>(a, s)
+                           : launch_level_t<64, 16, 12, FMA>(a, s);
 }
 
 static int plan_launches(hipacc_pyramid_plan* p, hipStream_t s) {
   for (int l = 0; l + 1 < p->levels; ++l) {
-    if (p->cnt[l] == 0) continue;
-    PyrLevelArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.imgs = p->d_imgs;
-    a.tiles = p->d_tiles + p->off[l];
-    a.ntiles = p->cnt[l];
-    a.level = l;
-    a.boundary = p->boundary;
-    std::memcpy(a.w, p->w, sizeof(a.w));
-    // one block per tile: the hardware keeps several tiles per CU in flight,
-    // hiding the staging latency of one behind the arithmetic of another
-    const int nb = a.ntiles;
-    int st;
-    if (p->fma) {
-      st = p->tile_rows == 8    ? launch_level_t<8, true>(a, nb, s)
-           : p->tile_rows == 32 ? launch_level_t<32, true>(a, nb, s)
-                                : launch_level_t<16, true>(a, nb, s);
-    } else {
-      st = p->tile_rows == 8    ? launch_level_t<8, false>(a, nb, s)
-           : p->tile_rows == 32 ? launch_level_t<32, false>(a, nb, s)
-                                : launch_level_t<16, false>(a, nb, s);
+    for (int wide = 1; wide >= 0; --wide) {
+      const int n = wide ? p->cntw[l] : p->cnt[l];
+      if (n == 0) continue;
+      PyrLevelArgs a;
+      std::memset(&a, 0, sizeof(a));
+      a.imgs = p->d_imgs;
+      a.tiles = p->d_tiles + (wide ? p->offw[l] : p->off[l]);
+      a.ntiles = n;
+      a.level = l;
+      a.boundary = p->boundary;
+      std::memcpy(a.w, p->w, sizeof(a.w));
+      const int st = p->fma ? launch_level<true>(a, wide != 0, p->tile_rows, s)
+                            : launch_level<false>(a, wide != 0, p->tile_rows, s);
+      if (st) return st;
     }
-    if (st) return st;
   }
   return HIPACC_OK;
 }
@@ -262,7 +294,9 @@ int hipacc_hip_pyramid_plan_create(const hipacc_pyramid_desc* d, void* stream,
   std::vector<PyrImage> imgs(d->num_images);
   p->off.assign(d->levels, 0);
   p->cnt.assign(d->levels, 0);
-  std::vector<std::vector<PyrTile>> lists(d->levels);
+  p->offw.assign(d->levels, 0);
+  p->cntw.assign(d->levels, 0);
+  std::vector<std::vector<PyrTile>> lists(d->levels), wlists(d->levels), nlists(d->levels);
   for (int i = 0; i < d->num_images; ++i) {
     const hipacc_pyramid_image& s = d->images[i];
     PyrImage& m = imgs[i];
@@ -285,11 +319,28 @@ int hipacc_hip_pyramid_plan_create(const hipacc_pyramid_desc* d, void* stream,
       if (l + 1 < d->levels) {
         m.lap[l] = s.lap[l];
         m.ls[l] = s.lap_stride[l];
+        // images at least 9 x 9 at this level (NN step <= 2.25) may take
+        // wide tiles; the smallest take narrow ones
+        if (w >= 9 && h >= 9) {
+          for (int y0 = 0; y0 < h; y0 += 32)
+            for (int x0 = 0; x0 < w; x0 += 128) wlists[l].push_back(PyrTile{i, x0, y0});
+        }
         for (int y0 = 0; y0 < h; y0 += p->tile_rows)
-          for (int x0 = 0; x0 < w; x0 += 64) lists[l].push_back(PyrTile{i, x0, y0});
+          for (int x0 = 0; x0 < w; x0 += 64) nlists[l].push_back(PyrTile{i, x0, y0});
       }
       w /= 2;
       h /= 2;
+    }
+  }
+  // A level takes wide tiles when they fill the device several times over;
+  // a smaller level takes narrow tiles for every image (more blocks).
+  for (int l = 0; l < d->levels; ++l) {
+    if ((int)wlists[l].size() >= kPyrWideMin) {
+      for (const PyrTile& t : nlists[l])
+        if (imgs[t.img].w[l] < 9 || imgs[t.img].h[l] < 9) lists[l].push_back(t);
+    } else {
+      wlists[l].clear();
+      lists[l] = nlists[l];
     }
   }
   std::vector<PyrTile> all;
@@ -297,6 +348,9 @@ int hipacc_hip_pyramid_plan_create(const hipacc_pyramid_desc* d, void* stream,
     p->off[l] = (int)all.size();
     p->cnt[l] = (int)lists[l].size();
     all.insert(all.end(), lists[l].begin(), lists[l].end());
+    p->offw[l] = (int)all.size();
+    p->cntw[l] = (int)wlists[l].size();
+    all.insert(all.end(), wlists[l].begin(), wlists[l].end());
   }
   hipError_t e = hipMalloc(&p->d_imgs, sizeof(PyrImage) * imgs.size());
   if (e == hipSuccess) e = hipMalloc(&p->d_tiles, sizeof(PyrTile) * (all.empty() ? 1 : all.size()));
