@@ -203,7 +203,7 @@ def bench_config4(op, capi, dev, steps):
 def bench_config5(op, capi, dev, reps):
     """64 float images, W ~ U{256..4096}, H ~ U{256..2160} (seeded), 8-level
     Gaussian / Laplacian pyramids (5x5 binomial, CLAMP): the batched plan
-    (one persistent launch per level over all images, replayed as a hipGraph)
+    (one launch per level over all images, replayed as a hipGraph)
     against the pyramid sample's per-image, per-level launches (convolution,
     NN subsample, LF difference), issued eagerly and also captured in a graph."""
     import torch

@@ -1,6 +1,6 @@
 // pyramid.hip -- resampling point operators (Interpolate::NN / LF) and the
-// batched Gaussian / Laplacian pyramid: one persistent launch per level over
-// a work list of 64-column tiles of every image, the launches captured once
+// batched Gaussian / Laplacian pyramid: one launch per level over a work list
+// of tiles of every image (one block per tile), the launches captured once
 // into a hipGraph.
 #include <hip/hip_runtime.h>
 

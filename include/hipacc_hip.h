@@ -370,8 +370,9 @@ int hipacc_hip_resample(const hipacc_resample_desc* desc, void* stream);
  *   L[l]   = G[l] - LF-upsample(G[l+1])          (LF accessor: CLAMP)
  * with the convolution as hipacc_hip_local (float SUM, row-major taps,
  * fp_contract as there).  L[levels-1] is G[levels-1] itself (not written).
- * A plan holds, for all images, one work list of 64x16 tiles per level and
- * executes each level as one persistent launch over that list (the blur is
+ * A plan holds, for all images, a work list of tiles per level (128x32 for
+ * the large levels, 64 x tile_rows for the small ones) and executes each
+ * level as one launch over that list, one block per tile (the blur is
  * evaluated only at the subsampled points, the upsampled G[l+1] comes from
  * LDS); the plan's launches are captured once into a hipGraph and replayed
  * by hipacc_hip_pyramid_run.  Results equal the three-kernels-per-level
@@ -394,7 +395,8 @@ typedef struct hipacc_pyramid_desc {
   const float* mask;                     /* host 5x5, NULL: (1 4 6 4 1)^T (1 4 6 4 1) / 256 */
   int32_t boundary;                      /* of the convolution (CONSTANT: 0) */
   int32_t fp_contract;
-  int32_t tile_rows;                     /* rows of L per tile: 0 = tuned/default (16), 8, 16, 32 */
+  int32_t tile_rows;                     /* rows of L per narrow tile (small levels):
+                                            0 = default (16), 8, 16, 32     */
 } hipacc_pyramid_desc;
 typedef struct hipacc_pyramid_plan hipacc_pyramid_plan;
 /* Builds the work lists (device memory, uploaded on `stream`, which is
