@@ -256,28 +256,32 @@ __global__ __launch_bounds__(256) void conv_dw_kernel(const ConvGradArgs a, int 
         ((red[k] + red[KK + k]) + red[2 * KK + k]) + red[3 * KK + k];
 }
 
-// dw by row streaming (no LDS tiles): a wave owns a strip of 62 lanes x 4
-// columns of dY and a segment of S rows; lanes 0 and 63 only supply the X
-// halo.  The wave walks the X rows the segment's windows touch (S + 2R rows)
-// and keeps the K dY rows each X row pairs with in a shifting register ring:
-// X row iy meets dY row iy + R - dy through tap row dy.  The X window of a
-// lane comes from its neighbours through DPP shifts.  Waves that touch the
-// image border read X through the boundary map (per-lane column tables,
-// scalar row remap).
-template <int R, int S>
-__global__ __launch_bounds__(256) void conv_dw_stream_kernel(const ConvGradArgs a, int strips,
-                                                             int segs, int total_waves) {
+// dw by row streaming: a wave owns a strip of 62 lanes x 4 columns of dY and
+// a segment of S rows; lanes 0 and 63 only supply the X halo.  The wave walks
+// the X rows the segment's windows touch (S + 2R rows) and keeps the K dY
+// rows each X row pairs with in a shifting register ring: X row iy meets dY
+// row iy + R - dy through tap row dy.  The X window of a lane comes from its
+// neighbours through DPP shifts.  Blocks are single waves.  Interior waves
+// stage their X and dY rows PF rows ahead in a wave-private LDS ring with
+// direct global->LDS loads (waits counted per wave, no barrier); waves that
+// touch the image border read X through the boundary map (per-lane column
+// tables, scalar row remap) one row ahead in registers.
+template <int R, int S, int PF>
+__global__ __launch_bounds__(64) void conv_dw_stream_kernel(const ConvGradArgs a, int strips,
+                                                            int segs, int total_waves) {
   constexpr int K = 2 * R + 1, KK = K * K, V = 4;
   constexpr int HL = (R + V - 1) / V;
   constexpr int NW = V + 2 * R;
   constexpr int STRIP = (64 - 2 * HL) * V;
-  __shared__ float red[4 * KK];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int NSL = PF + 1;                          // LDS ring slots
+  __shared__ __align__(16) float4 xs[NSL * 64];
+  __shared__ __align__(16) float4 gls[NSL * 64];
+  const int lane = threadIdx.x;
   float dw[KK];
 #pragma unroll
   for (int t = 0; t < KK; ++t) dw[t] = 0.0f;
 
-  for (int wv = blockIdx.x * 4 + wave; wv < total_waves; wv += gridDim.x * 4) {
+  for (int wv = blockIdx.x; wv < total_waves; wv += gridDim.x) {
     const int per_img = strips * segs;
     const int b = wv / per_img;
     const int rem = wv - b * per_img;
@@ -292,73 +296,21 @@ __global__ __launch_bounds__(256) void conv_dw_stream_kernel(const ConvGradArgs 
     const int nx = ny + 2 * R;                         // X rows streamed
     const bool fast = a.vec_ok && x_wave >= 0 && x_wave + 64 * V <= a.w && y0 - R >= 0 &&
                       y0 + ny + R <= a.h;
-    Geom g;
-    g.in = X;
-    g.in_stride = a.x_stride;
-    g.lo_x = 0;
-    g.hi_x = a.w;
-    g.lo_y = 0;
-    g.hi_y = a.h;
-    g.in_imgw = a.w;
-    g.in_imgh = a.h;
-    g.boundary = a.boundary;
-    g.const_f = a.cval;
-    g.const_i = 0;
-    g.in_dtype = HIPACC_F32;
-    const LaneCols<V> cols = lane_cols<V>(g, xl);
-
-    auto load_x = [&](float (&v)[V], int k) {
-      const int iy = y0 - R + k;
-      if (fast) {
-        load_px<float, V>(v, X + (long long)iy * a.x_stride + xl);
-      } else {
-        bool rc;
-        const int y = row_src(g, iy, rc);
-        const float* row = X + (long long)y * a.x_stride;
-#pragma unroll
-        for (int e = 0; e < V; ++e) {
-          const float t = row[cols.x[e]];
-          v[e] = (rc || cols.cst[e]) ? a.cval : t;
-        }
-      }
-    };
-    auto load_g = [&](float (&v)[V], int k) {
-      const int y = y0 + k;
-#pragma unroll
-      for (int e = 0; e < V; ++e) v[e] = 0.0f;
-      if (k >= ny || !own) return;
-      const float* row = G + (long long)y * a.gy_stride;
-      if (fast) {
-        load_px<float, V>(v, row + xl);
-      } else {
-#pragma unroll
-        for (int e = 0; e < V; ++e)
-          if (xl + e < a.w) v[e] = row[xl + e];
-      }
-    };
 
     float ring[K][V];
 #pragma unroll
     for (int d = 0; d < K; ++d)
 #pragma unroll
       for (int e = 0; e < V; ++e) ring[d][e] = 0.0f;
-    float xn[V], gn[V];
-    load_x(xn, 0);
-    load_g(gn, 0);
-    for (int k = 0; k < nx; ++k) {
-      float cur[V];
-#pragma unroll
-      for (int e = 0; e < V; ++e) cur[e] = xn[e];
+    // X row k (absolute y0 - R + k) pairs with the ring of dY rows; the
+    // newest ring entry is dY row y0 + k (zero past the segment)
+    auto step = [&](const float (&cur)[V], const float (&gn)[V], int k) {
 #pragma unroll
       for (int d = K - 1; d > 0; --d)
 #pragma unroll
         for (int e = 0; e < V; ++e) ring[d][e] = ring[d - 1][e];
 #pragma unroll
       for (int e = 0; e < V; ++e) ring[0][e] = gn[e];
-      if (k + 1 < nx) {
-        load_x(xn, k + 1);
-        load_g(gn, k + 1);
-      }
       float win[NW];
 #pragma unroll
       for (int wi = 0; wi < NW; ++wi) {
@@ -378,18 +330,94 @@ __global__ __launch_bounds__(256) void conv_dw_stream_kernel(const ConvGradArgs 
           for (int j = 0; j < V; ++j)
             dw[dy * K + dx] = __builtin_fmaf(ring[dy][j], win[j + dx], dw[dy * K + dx]);
       }
+    };
+
+    if (fast) {
+      // chunk k: X row y0 - R + k and dY row y0 + min(k, ny - 1) into slot
+      // k % NSL; the rows past the stream are clamped (their values unused)
+      auto issue = [&](int k) {
+        const int slot = k % NSL;
+        const int kx = min(k, nx - 1), kg = min(k, ny - 1);
+        __builtin_amdgcn_global_load_lds(
+            static_cast<const void*>(X + (long long)(y0 - R + kx) * a.x_stride + xl),
+            (__attribute__((address_space(3))) void*)(xs + slot * 64), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds(
+            static_cast<const void*>(G + (long long)(y0 + kg) * a.gy_stride + xl),
+            (__attribute__((address_space(3))) void*)(gls + slot * 64), 16, 0, 0);
+      };
+#pragma unroll
+      for (int k = 0; k < PF; ++k) issue(k);
+      for (int k = 0; k < nx; ++k) {
+        // rows k+1 .. k+PF-1 (2 loads each) may still be in flight
+        asm volatile("s_waitcnt vmcnt(%0)" ::"n"(2 * (PF - 1)) : "memory");
+        const float4 xq = xs[(k % NSL) * 64 + lane];
+        const float4 gq = gls[(k % NSL) * 64 + lane];
+        const bool gv = own && k < ny;
+        const float cur[V] = {xq.x, xq.y, xq.z, xq.w};
+        const float gn[V] = {gv ? gq.x : 0.0f, gv ? gq.y : 0.0f, gv ? gq.z : 0.0f,
+                             gv ? gq.w : 0.0f};
+        issue(k + PF);   // into the slot row k-1 used
+        step(cur, gn, k);
+      }
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // the ring is reused next
+    } else {
+      Geom g;
+      g.in = X;
+      g.in_stride = a.x_stride;
+      g.lo_x = 0;
+      g.hi_x = a.w;
+      g.lo_y = 0;
+      g.hi_y = a.h;
+      g.in_imgw = a.w;
+      g.in_imgh = a.h;
+      g.boundary = a.boundary;
+      g.const_f = a.cval;
+      g.const_i = 0;
+      g.in_dtype = HIPACC_F32;
+      const LaneCols<V> cols = lane_cols<V>(g, xl);
+      auto load_x = [&](float (&v)[V], int k) {
+        bool rc;
+        const int y = row_src(g, y0 - R + k, rc);
+        const float* row = X + (long long)y * a.x_stride;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          const float t = row[cols.x[e]];
+          v[e] = (rc || cols.cst[e]) ? a.cval : t;
+        }
+      };
+      auto load_g = [&](float (&v)[V], int k) {
+#pragma unroll
+        for (int e = 0; e < V; ++e) v[e] = 0.0f;
+        if (k >= ny || !own) return;
+        const float* row = G + (long long)(y0 + k) * a.gy_stride;
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+          if (xl + e < a.w) v[e] = row[xl + e];
+      };
+      float xn[V], gn[V];
+      load_x(xn, 0);
+      load_g(gn, 0);
+      for (int k = 0; k < nx; ++k) {
+        float cur[V], gk[V];
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          cur[e] = xn[e];
+          gk[e] = gn[e];
+        }
+        if (k + 1 < nx) {
+          load_x(xn, k + 1);
+          load_g(gn, k + 1);
+        }
+        step(cur, gk, k);
+      }
     }
   }
 
 #pragma unroll
   for (int t = 0; t < KK; ++t) {
     const float v = wave_sum(dw[t]);
-    if (lane == 0) red[wave * KK + t] = v;
+    if (lane == 0) a.partials[(long long)blockIdx.x * KK + t] = v;
   }
-  __syncthreads();
-  for (int t = threadIdx.x; t < KK; t += 256)
-    a.partials[(long long)blockIdx.x * KK + t] =
-        ((red[t] + red[KK + t]) + red[2 * KK + t]) + red[3 * KK + t];
 }
 
 // Any mask shape: block (chunk, tap) sums dY(p) * X_bh(p + t - r) over the
@@ -444,16 +472,16 @@ int launch_conv_dw(const ConvGradArgs& a0, float* dw, hipStream_t s) {
   const int kk = a.kx * a.ky;
   int nblocks;
   if (a.kx == a.ky && (a.kx == 3 || a.kx == 5 || a.kx == 7 || a.kx == 9) && a.dw_variant != kVarTile) {
-    constexpr int S = 16;
+    constexpr int S = 32, PF = 8;
     const int strip = (64 - 2) * 4;   // HL = 1 for R <= 4
     const int strips = ceil_div(a.w, strip), segs = ceil_div(a.h, S);
     const int waves = strips * segs * a.batch;
-    nblocks = min(ceil_div(waves, 4), kGradBlocks);
+    nblocks = min(waves, kGradBlocks);
     switch (a.kx) {
-      case 3: hipLaunchKernelGGL((conv_dw_stream_kernel<1, S>), dim3(nblocks), dim3(256), 0, s, a, strips, segs, waves); break;
-      case 5: hipLaunchKernelGGL((conv_dw_stream_kernel<2, S>), dim3(nblocks), dim3(256), 0, s, a, strips, segs, waves); break;
-      case 7: hipLaunchKernelGGL((conv_dw_stream_kernel<3, S>), dim3(nblocks), dim3(256), 0, s, a, strips, segs, waves); break;
-      default: hipLaunchKernelGGL((conv_dw_stream_kernel<4, S>), dim3(nblocks), dim3(256), 0, s, a, strips, segs, waves); break;
+      case 3: hipLaunchKernelGGL((conv_dw_stream_kernel<1, S, PF>), dim3(nblocks), dim3(64), 0, s, a, strips, segs, waves); break;
+      case 5: hipLaunchKernelGGL((conv_dw_stream_kernel<2, S, PF>), dim3(nblocks), dim3(64), 0, s, a, strips, segs, waves); break;
+      case 7: hipLaunchKernelGGL((conv_dw_stream_kernel<3, S, PF>), dim3(nblocks), dim3(64), 0, s, a, strips, segs, waves); break;
+      default: hipLaunchKernelGGL((conv_dw_stream_kernel<4, S, PF>), dim3(nblocks), dim3(64), 0, s, a, strips, segs, waves); break;
     }
   } else if (a.kx == a.ky && (a.kx == 3 || a.kx == 5 || a.kx == 7 || a.kx == 9)) {
     a.tiles_x = ceil_div(a.w, 64);
