@@ -10,9 +10,10 @@ constexpr int kRingChunk = 8;   // rows per LDS chunk
 constexpr int kRingSets = 2;    // register sets of staged rows in flight (edge strips)
 constexpr int kRingDepth = 1;   // chunks of direct loads in flight (interior strips)
 
-// Blocks resident on the whole device at once, at most 4 per CU: on a CU
-// the oldest block is served first, so each further block lengthens the
-// tail in which the youngest finishes alone.
+// Blocks resident on the whole device at once, at most 3 per CU (4 for
+// 7x7): on a CU the oldest block is served first, so each further block
+// lengthens the tail in which the youngest finishes alone (3 measured faster
+// than 4 for 3x3 and 5x5, 4 faster for 7x7; 9x9 fits 3).
 template <int K, bool FMA, bool WDEV>
 static int ring_slots() {
   static const int slots = [] {
@@ -24,7 +25,8 @@ static int ring_slots() {
             reinterpret_cast<const void*>(&ring_kernel<K, FMA, WDEV, kRingChunk, kRingSets, kRingDepth>), 256,
             0) != hipSuccess)
       return 0;
-    return cus * (per_cu < 4 ? per_cu : 4);
+    const int cap = K == 7 ? 4 : 3;
+    return cus * (per_cu < cap ? per_cu : cap);
   }();
   return slots;
 }
@@ -32,7 +34,7 @@ static int ring_slots() {
 // Time of a row of an edge strip (per-pixel loads and stores) relative to a
 // row of an interior strip, measured at 4096^2: the VALU work of the larger
 // masks hides most of the extra memory instructions.
-static float ring_edge_cost(int k) { return k <= 5 ? 1.55f : 1.15f; }
+static float ring_edge_cost(int k) { return k <= 5 ? 1.4f : 1.15f; }
 
 // Classifies the strips the way ring_kernel does (edge: the strip's staged
 // columns leave the raw-readable region, or its stored columns are not all
@@ -58,8 +60,14 @@ static RingGeom ring_geom(const LocalArgs& a, int k, int seg_rows, int slots) {
   while (rg.nl + rg.nr < rg.strips && edge(rg.strips - 1 - rg.nr)) ++rg.nr;
   const int ne = rg.nl + rg.nr, nm = rg.strips - ne;
   const float ce = (nm > 0 && ne > 0) ? ring_edge_cost(k) : 1.0f;
-  auto size = [&](int rows) {
+  // chosen segment heights are whole chunks: a partial last chunk costs a
+  // full step
+  auto size = [&](int rows, bool whole) {
     int re = (int)((float)rows / ce);
+    if (whole) {
+      rows = ceil_div(rows, kRingChunk) * kRingChunk;
+      re = ceil_div((int)((float)rows / ce), kRingChunk) * kRingChunk;
+    }
     if (re < 8) re = 8;
     rg.seg_rows = rows < g.is_h ? rows : g.is_h;
     rg.seg_rows_e = re < g.is_h ? re : g.is_h;
@@ -69,17 +77,17 @@ static RingGeom ring_geom(const LocalArgs& a, int k, int seg_rows, int slots) {
     rg.items = rg.per_img * g.batch;
   };
   if (seg_rows > 0) {
-    size(seg_rows);
+    size(seg_rows, false);
   } else {
     // interior-row units of work, spread over the slots; then taller
     // segments until the rounding leaves no item for a second round
     const double units = ((double)nm + ne * ce) * g.is_h * g.batch;
     int rows = slots > 0 ? (int)std::ceil(units / slots) : g.is_h;
     if (rows < 16) rows = 16;
-    size(rows);
+    size(rows, true);
     for (int it = 0; it < 64 && slots > 0 && rg.items > slots && rows < g.is_h; ++it) {
       rows += rows / 32 + 1;
-      size(rows);
+      size(rows, true);
     }
   }
   return rg;
