@@ -405,4 +405,213 @@ __global__ __launch_bounds__(256) void ring_kernel(const LocalArgs a, const Ring
                                               it.nout);
 }
 
+// ---------------------------------------------------------------------------
+// Separable masks (f32 -> f32).  The strip keeps HL halo lanes on each side
+// (HL = ceil(RX / 4)), so 64 - 2 HL lanes are stored.  Each wave stages CH/4
+// input rows of a chunk in registers (NS sets in flight), computes their
+// horizontal pass (taps from the neighbour lanes through DPP) and writes the
+// intermediate rows into a 3-chunk LDS ring; after the barrier each wave
+// sums KY intermediate rows per output row.  The arithmetic equals the
+// streaming separable kernel's (hh_stream.hpp) operation for operation: the
+// horizontal sum in tap order, the intermediate row replaced by the
+// constant for a row outside the window in CONSTANT mode, then the vertical
+// sum in tap order, each sum starting with a plain product.
+// ---------------------------------------------------------------------------
+template <int KX, int KY, bool FMA, bool EDGE, int CH, int NS>
+__device__ __forceinline__ void ring_sep_body(const SepArgs& a, float4* __restrict__ lds,
+                                              const float* __restrict__ in,
+                                              float* __restrict__ out, int xl, int y0, int nout) {
+  constexpr int V = kRingV, QR = CH / 4;
+  constexpr int RX = KX / 2, RY = KY / 2;
+  constexpr int HL = (RX + V - 1) / V;
+  constexpr int NT = QR + KY - 1;
+  constexpr int NWIN = V + 2 * RX;
+  static_assert(2 * RY <= CH, "vertical radius exceeds the ring's halo");
+  const Geom& g = a.g;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nin = nout + KY - 1;
+  const int iy0 = g.in_oy + y0 - RY;
+  const int ix = g.in_ox + xl;
+  const bool cmode = g.boundary == HIPACC_BOUNDARY_CONSTANT;
+  const float cv = g.const_f;
+  int eoff[V];
+  bool ecst[V];
+  if constexpr (EDGE) {
+    const LaneCols<V> cols = lane_cols<V>(g, ix);
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      ecst[e] = cols.cst[e];
+      eoff[e] = cols.cst[e] ? kRingDrop : cols.x[e] * 4;
+    }
+  }
+  const int in_row_bytes = g.in_imgw * 4;
+
+  auto load_chunk = [&](int c) {
+    RingStage<false, QR> st;
+    st.rc = 0;
+#pragma unroll
+    for (int i = 0; i < QR; ++i) {
+      const int k = min(c * CH + wave + 4 * i, nin - 1);
+      bool rc;
+      const int y = row_src(g, iy0 + k, rc);
+      st.rc |= rc ? 1 << i : 0;
+      const float* row = in + (long long)y * g.in_stride;
+      if constexpr (!EDGE) {
+        st.r[i] = *reinterpret_cast<const float4*>(row + ix);
+      } else {
+        const __amdgpu_buffer_rsrc_t rs = ring_rsrc(row, in_row_bytes);
+        float v[V];
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+          v[e] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, eoff[e], 0, 0));
+        st.r[i] = make_float4(v[0], v[1], v[2], v[3]);
+      }
+    }
+    return st;
+  };
+  // horizontal pass of the staged rows, written to LDS slot c % 3
+  auto write_chunk = [&](int c, const RingStage<false, QR>& st) {
+    const int slot = c % 3;
+#pragma unroll
+    for (int i = 0; i < QR; ++i) {
+      float cur[V] = {st.r[i].x, st.r[i].y, st.r[i].z, st.r[i].w};
+      if constexpr (EDGE) {
+        if (cmode) {
+#pragma unroll
+          for (int e = 0; e < V; ++e) cur[e] = ecst[e] ? cv : cur[e];
+        }
+      }
+      float win[NWIN];
+#pragma unroll
+      for (int wi = 0; wi < NWIN; ++wi) {
+        const int rel = wi - RX;
+        const int d = rel >= 0 ? rel / V : -((-rel + V - 1) / V);
+        const int e = rel - d * V;
+        win[wi] = d == 0 ? cur[e] : lane_shift(cur[e], d);
+      }
+      float h[V];
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+        float s0 = 0.0f;
+#pragma unroll
+        for (int dx = 0; dx < KX; ++dx)
+          s0 = dx == 0 ? mul_first(a.rowf[dx], win[e]) : madd<FMA>(a.rowf[dx], win[e + dx], s0);
+        h[e] = s0;
+      }
+      if (cmode && ((st.rc >> i) & 1)) {
+#pragma unroll
+        for (int e = 0; e < V; ++e) h[e] = cv;
+      }
+      lds[(slot * CH + wave + 4 * i) * 64 + lane] = make_float4(h[0], h[1], h[2], h[3]);
+    }
+  };
+
+  const bool store_lane = lane >= HL && lane < 64 - HL;
+  const bool full = store_lane && (!EDGE || (a.out_vec_ok && xl + V <= g.is_w));
+  const int voff = full ? xl * 4 : kRingDrop;
+  int soff[V];
+  if constexpr (EDGE) {
+#pragma unroll
+    for (int e = 0; e < V; ++e)
+      soff[e] = (store_lane && !full && xl + e < g.is_w) ? (xl + e) * 4 : kRingDrop;
+  }
+  float* obase = out + (long long)(g.out_oy + y0) * g.out_stride + g.out_ox;
+  const int out_row_bytes = g.is_w * 4;
+
+  auto compute = [&](int s) {
+    const int jw = wave * QR;
+    const int j0 = s * CH + jw;
+    float acc[QR][V];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int off = jw + t;
+      const int c = off >= CH ? 1 : 0;
+      const int slot = (s + c) % 3;
+      const float4 q = lds[(slot * CH + off - c * CH) * 64 + lane];
+      const float hv[V] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+      for (int qq = 0; qq < QR; ++qq) {
+        const int dyi = t - qq;
+        if (dyi < 0 || dyi >= KY) continue;
+        const float w = a.colf[dyi];
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+          acc[qq][e] = dyi == 0 ? mul_first(w, hv[e]) : madd<FMA>(w, hv[e], acc[qq][e]);
+      }
+    }
+#pragma unroll
+    for (int qq = 0; qq < QR; ++qq) {
+      const int j = j0 + qq;
+      const __amdgpu_buffer_rsrc_t rs =
+          ring_rsrc(obase + (long long)j * g.out_stride, j < nout ? out_row_bytes : 0);
+      const ring_u4 v = {__builtin_bit_cast(unsigned, acc[qq][0]),
+                         __builtin_bit_cast(unsigned, acc[qq][1]),
+                         __builtin_bit_cast(unsigned, acc[qq][2]),
+                         __builtin_bit_cast(unsigned, acc[qq][3])};
+      __builtin_amdgcn_raw_buffer_store_b128(v, rs, voff, 0, 2);
+      if constexpr (EDGE) {
+#pragma unroll
+        for (int e = 0; e < V; ++e) __builtin_amdgcn_raw_buffer_store_b32(v[e], rs, soff[e], 0, 2);
+      }
+    }
+  };
+
+  const int nos = (nout + CH - 1) / CH;
+  RingStage<false, QR> sets[NS];
+  {
+    const RingStage<false, QR> c0 = load_chunk(0), c1 = load_chunk(1);
+    write_chunk(0, c0);
+    write_chunk(1, c1);
+  }
+#pragma unroll
+  for (int u = 0; u < NS; ++u) sets[(u + 2) % NS] = load_chunk(u + 2);
+  __syncthreads();
+  int s = 0;
+  for (; s + NS <= nos; s += NS) {
+#pragma unroll
+    for (int u = 0; u < NS; ++u) {
+      const int k = (u + 2) % NS;
+      write_chunk(s + u + 2, sets[k]);
+      sets[k] = load_chunk(s + u + 2 + NS);
+      compute(s + u);
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < NS; ++u) {
+    if (s + u < nos) {
+      const int k = (u + 2) % NS;
+      write_chunk(s + u + 2, sets[k]);
+      compute(s + u);
+      __syncthreads();
+    }
+  }
+}
+
+template <int KX, int KY, bool FMA, int CH, int NS>
+__global__ __launch_bounds__(256) void ring_sep_kernel(const SepArgs a, const RingGeom rg) {
+  constexpr int HL = (KX / 2 + kRingV - 1) / kRingV;
+  constexpr int SW = (64 - 2 * HL) * kRingV;
+  __shared__ __align__(16) float4 lds[3 * CH * 64];
+  const Geom& g = a.g;
+  if ((int)blockIdx.x >= rg.items) return;
+  const RingItem it = ring_item(rg, g, blockIdx.x);
+  const float* in = static_cast<const float*>(g.in) + (long long)it.b * g.in_plane;
+  float* out = static_cast<float*>(g.out) + (long long)it.b * g.out_plane;
+  const int lane = threadIdx.x & 63;
+  const int xs = it.strip * SW - HL * kRingV;
+  int lx, hx, ly, hy;
+  raw_region(g, lx, hx, ly, hy);
+  const int ix0 = g.in_ox + xs;
+  const bool raw = a.in_vec_ok && a.out_vec_ok && ix0 >= lx && ix0 + 64 * kRingV <= hx &&
+                   xs + (64 - HL) * kRingV <= g.is_w;
+  if (raw)
+    ring_sep_body<KX, KY, FMA, false, CH, NS>(a, lds, in, out, xs + lane * kRingV, it.y0,
+                                             it.nout);
+  else
+    ring_sep_body<KX, KY, FMA, true, CH, NS>(a, lds, in, out, xs + lane * kRingV, it.y0,
+                                            it.nout);
+}
+
 }  // namespace hh

@@ -1,5 +1,6 @@
 // ring_2d.hip -- instantiations and launch geometry of the block row ring
-// (hh_ring.hpp): f32 -> f32 weighted SUM, square masks 3x3 .. 9x9.
+// (hh_ring.hpp): f32 -> f32 weighted SUM, square masks 3x3 .. 9x9, and
+// separable f32 masks 3 .. 13 taps.
 #include <cmath>
 
 #include "hh_ring.hpp"
@@ -36,37 +37,39 @@ static int ring_slots() {
 // masks hides most of the extra memory instructions.
 static float ring_edge_cost(int k) { return k <= 5 ? 1.4f : 1.15f; }
 
-// Classifies the strips the way ring_kernel does (edge: the strip's staged
-// columns leave the raw-readable region, or its stored columns are not all
-// full vectors) and sizes the segments.  seg_rows > 0 forces the interior
-// segment height; 0 picks it so that one round of items fills `slots`.
-static RingGeom ring_geom(const LocalArgs& a, int k, int seg_rows, int slots) {
-  const Geom& g = a.g;
+// Classifies the strips the way ring_kernel / ring_sep_kernel do (edge: the
+// strip's staged columns leave the raw-readable region, or its stored
+// columns are not all full vectors; hl halo lanes per side) and sizes the
+// segments.  seg_rows > 0 forces the interior segment height; 0 picks it so
+// that one round of items fills `slots`.
+static RingGeom ring_geom(const Geom& g, int in_vec_ok, int out_vec_ok, float edge_cost, int hl,
+                          int chunk, int seg_rows, int slots) {
   RingGeom rg;
-  rg.strips = ceil_div(g.is_w, kRingStripW);
+  const int sw = (64 - 2 * hl) * kRingV;
+  rg.strips = ceil_div(g.is_w, sw);
   int lx = g.lo_x, hx = g.hi_x;
   if (g.boundary == HIPACC_BOUNDARY_UNDEFINED) {
     lx = 0;
     hx = g.in_imgw;
   }
   auto edge = [&](int strip) {
-    const int xs = strip * kRingStripW - kRingV, ix0 = g.in_ox + xs;
-    return !(a.in_vec_ok && a.out_vec_ok && ix0 >= lx && ix0 + 64 * kRingV <= hx &&
-             xs + 63 * kRingV <= g.is_w);
+    const int xs = strip * sw - hl * kRingV, ix0 = g.in_ox + xs;
+    return !(in_vec_ok && out_vec_ok && ix0 >= lx && ix0 + 64 * kRingV <= hx &&
+             xs + (64 - hl) * kRingV <= g.is_w);
   };
   rg.nl = 0;
   while (rg.nl < rg.strips && edge(rg.nl)) ++rg.nl;
   rg.nr = 0;
   while (rg.nl + rg.nr < rg.strips && edge(rg.strips - 1 - rg.nr)) ++rg.nr;
   const int ne = rg.nl + rg.nr, nm = rg.strips - ne;
-  const float ce = (nm > 0 && ne > 0) ? ring_edge_cost(k) : 1.0f;
+  const float ce = (nm > 0 && ne > 0) ? edge_cost : 1.0f;
   // chosen segment heights are whole chunks: a partial last chunk costs a
   // full step
   auto size = [&](int rows, bool whole) {
     int re = (int)((float)rows / ce);
     if (whole) {
-      rows = ceil_div(rows, kRingChunk) * kRingChunk;
-      re = ceil_div((int)((float)rows / ce), kRingChunk) * kRingChunk;
+      rows = ceil_div(rows, chunk) * chunk;
+      re = ceil_div((int)((float)rows / ce), chunk) * chunk;
     }
     if (re < 8) re = 8;
     rg.seg_rows = rows < g.is_h ? rows : g.is_h;
@@ -101,10 +104,56 @@ static int launch_ring(const LocalArgs& a0, int seg_rows, hipStream_t s) {
   if (strips * a.g.is_h * a.g.batch > (1ll << 31))
     return set_error(HIPACC_ERR_UNSUPPORTED, "ring kernel: %lld strip rows",
                      strips * a.g.is_h * a.g.batch);
-  const RingGeom rg = ring_geom(a, K, seg_rows, ring_slots<K, FMA, WDEV>());
+  const RingGeom rg = ring_geom(a.g, a.in_vec_ok, a.out_vec_ok, ring_edge_cost(K), 1, kRingChunk,
+                                seg_rows, ring_slots<K, FMA, WDEV>());
   hipLaunchKernelGGL((ring_kernel<K, FMA, WDEV, kRingChunk, kRingSets, kRingDepth>), dim3(rg.items), dim3(256),
                      0, s, a, rg);
   return hip_status(hipGetLastError(), "ring_kernel launch");
+}
+
+// Separable masks: 16-row chunks (the vertical radius reaches 8), as many
+// resident blocks as fit (the ring takes 48 KB of LDS).
+constexpr int kRingSepChunk = 16;
+
+template <int K, bool FMA>
+static int launch_ring_sep(const SepArgs& a0, int seg_rows, hipStream_t s) {
+  constexpr int HL = (K / 2 + kRingV - 1) / kRingV;
+  static const int slots = [] {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(&ring_sep_kernel<K, K, FMA, kRingSepChunk, 2>),
+            256, 0) != hipSuccess)
+      return 0;
+    return cus * (per_cu < 3 ? per_cu : 3);
+  }();
+  SepArgs a = a0;
+  stream_align(a.g, kRingV, a.in_vec_ok, a.out_vec_ok);
+  const long long strips = ceil_div(a.g.is_w, (64 - 2 * HL) * kRingV);
+  if (strips * a.g.is_h * a.g.batch > (1ll << 31))
+    return set_error(HIPACC_ERR_UNSUPPORTED, "ring_sep kernel: %lld strip rows",
+                     strips * a.g.is_h * a.g.batch);
+  const RingGeom rg = ring_geom(a.g, a.in_vec_ok, a.out_vec_ok, 1.3f, HL, kRingSepChunk,
+                                seg_rows, slots);
+  hipLaunchKernelGGL((ring_sep_kernel<K, K, FMA, kRingSepChunk, 2>), dim3(rg.items), dim3(256),
+                     0, s, a, rg);
+  return hip_status(hipGetLastError(), "ring_sep_kernel launch");
+}
+
+int dispatch_ring_sep(const SepArgs& a, int acc_is_float, int fma, int seg_rows, hipStream_t s) {
+  const Geom& g = a.g;
+  if (!acc_is_float || a.kx != a.ky || g.in_dtype != HIPACC_F32 || g.out_dtype != HIPACC_F32 ||
+      g.post_div != 0.0f || g.cvt == HIPACC_CVT_ROUND)
+    return HIPACC_ERR_UNSUPPORTED;
+  if ((long long)g.in_imgw * 4 >= (1ll << 31) || (long long)g.is_w * 4 >= (1ll << 31))
+    return HIPACC_ERR_UNSUPPORTED;
+#define HH_RSEP(K)                                                                     \
+  if (a.kx == K) return fma ? launch_ring_sep<K, true>(a, seg_rows, s)                 \
+                            : launch_ring_sep<K, false>(a, seg_rows, s);
+  HH_RSEP(3) HH_RSEP(5) HH_RSEP(7) HH_RSEP(9) HH_RSEP(13)
+#undef HH_RSEP
+  return HIPACC_ERR_UNSUPPORTED;
 }
 
 int dispatch_ring_local(const LocalArgs& a, int seg_rows, hipStream_t s) {

@@ -215,11 +215,13 @@ static int launch_sep_variant(const SepArgs& a, int acc_is_float, int fma, hipSt
     case kVarStreamWide: return dispatch_stream_sep(a, acc_is_float, fma, param, 1, s);
     case kVarTile: return dispatch_sep(a, acc_is_float, fma, param, s);
     case kVarGeneric: return launch_sep_generic(a, acc_is_float, fma, s);
+    case kVarRing: return dispatch_ring_sep(a, acc_is_float, fma, param, s);
   }
   return HIPACC_ERR_UNSUPPORTED;
 }
 
-// variant kVarAuto: the tuned choice if any, else stream -> tile -> generic.
+// variant kVarAuto: the tuned choice if any, else ring -> stream -> tile ->
+// generic.
 int launch_separable(const SepArgs& a, int acc_is_float, int fp_contract, hipStream_t s,
                      int variant, int param) {
   if (variant != kVarAuto) return launch_sep_variant(a, acc_is_float, fp_contract, s, variant, param);
@@ -230,7 +232,8 @@ int launch_separable(const SepArgs& a, int acc_is_float, int fp_contract, hipStr
     const int st = launch_sep_variant(a, acc_is_float, fp_contract, s, v.variant, v.param);
     if (st != HIPACC_ERR_UNSUPPORTED) return st;
   }
-  int st = launch_sep_variant(a, acc_is_float, fp_contract, s, kVarStream, 0);
+  int st = launch_sep_variant(a, acc_is_float, fp_contract, s, kVarRing, 0);
+  if (st == HIPACC_ERR_UNSUPPORTED) st = launch_sep_variant(a, acc_is_float, fp_contract, s, kVarStream, 0);
   if (st == HIPACC_ERR_UNSUPPORTED) st = launch_sep_variant(a, acc_is_float, fp_contract, s, kVarTile, 0);
   if (st == HIPACC_ERR_UNSUPPORTED) st = launch_sep_generic(a, acc_is_float, fp_contract, s);
   return st;
