@@ -406,7 +406,7 @@ __global__ __launch_bounds__(256) void ring_kernel(const LocalArgs a, const Ring
 }
 
 // ---------------------------------------------------------------------------
-// Separable masks (f32 -> f32).  The strip keeps HL halo lanes on each side
+// Separable masks (f32 -> f32, u8 -> u8).  The strip keeps HL halo lanes on each side
 // (HL = ceil(RX / 4)), so 64 - 2 HL lanes are stored.  Each wave stages CH/4
 // input rows of a chunk in registers (NS sets in flight), computes their
 // horizontal pass (taps from the neighbour lanes through DPP) and writes the
@@ -417,15 +417,27 @@ __global__ __launch_bounds__(256) void ring_kernel(const LocalArgs a, const Ring
 // constant for a row outside the window in CONSTANT mode, then the vertical
 // sum in tap order, each sum starting with a plain product.
 // ---------------------------------------------------------------------------
-template <int KX, int KY, bool FMA, bool EDGE, int CH, int NS>
+// Staged input rows of the separable ring: 4 pixels per lane and row.
+template <typename TI, int QR> struct RingSepStage {
+  float4 r[QR];
+  int rc;
+};
+template <int QR> struct RingSepStage<uint8_t, QR> {
+  unsigned r[QR];   // 4 packed pixels
+  int rc;
+};
+
+template <typename TI, typename TO, int KX, int KY, bool FMA, bool EDGE, int CH, int NS>
 __device__ __forceinline__ void ring_sep_body(const SepArgs& a, float4* __restrict__ lds,
-                                              const float* __restrict__ in,
-                                              float* __restrict__ out, int xl, int y0, int nout) {
+                                              const TI* __restrict__ in, TO* __restrict__ out,
+                                              int xl, int y0, int nout) {
   constexpr int V = kRingV, QR = CH / 4;
   constexpr int RX = KX / 2, RY = KY / 2;
   constexpr int HL = (RX + V - 1) / V;
   constexpr int NT = QR + KY - 1;
   constexpr int NWIN = V + 2 * RX;
+  constexpr bool U8 = std::is_same<TI, uint8_t>::value;
+  constexpr int BI = (int)sizeof(TI), BO = (int)sizeof(TO);
   static_assert(2 * RY <= CH, "vertical radius exceeds the ring's halo");
   const Geom& g = a.g;
   const int lane = threadIdx.x & 63;
@@ -434,7 +446,9 @@ __device__ __forceinline__ void ring_sep_body(const SepArgs& a, float4* __restri
   const int iy0 = g.in_oy + y0 - RY;
   const int ix = g.in_ox + xl;
   const bool cmode = g.boundary == HIPACC_BOUNDARY_CONSTANT;
-  const float cv = g.const_f;
+  const float cpx = (float)const_pixel<TI>(g);   // a pixel outside the window
+  const float crow = g.const_f;                  // an intermediate row outside the window
+  const float rnd = g.cvt == HIPACC_CVT_ROUND ? 0.5f : 0.0f;
   int eoff[V];
   bool ecst[V];
   if constexpr (EDGE) {
@@ -442,13 +456,13 @@ __device__ __forceinline__ void ring_sep_body(const SepArgs& a, float4* __restri
 #pragma unroll
     for (int e = 0; e < V; ++e) {
       ecst[e] = cols.cst[e];
-      eoff[e] = cols.cst[e] ? kRingDrop : cols.x[e] * 4;
+      eoff[e] = cols.cst[e] ? kRingDrop : cols.x[e] * BI;
     }
   }
-  const int in_row_bytes = g.in_imgw * 4;
+  const int in_row_bytes = g.in_imgw * BI;
 
   auto load_chunk = [&](int c) {
-    RingStage<false, QR> st;
+    RingSepStage<TI, QR> st;
     st.rc = 0;
 #pragma unroll
     for (int i = 0; i < QR; ++i) {
@@ -456,30 +470,48 @@ __device__ __forceinline__ void ring_sep_body(const SepArgs& a, float4* __restri
       bool rc;
       const int y = row_src(g, iy0 + k, rc);
       st.rc |= rc ? 1 << i : 0;
-      const float* row = in + (long long)y * g.in_stride;
+      const TI* row = in + (long long)y * g.in_stride;
       if constexpr (!EDGE) {
-        st.r[i] = *reinterpret_cast<const float4*>(row + ix);
+        if constexpr (U8) st.r[i] = *reinterpret_cast<const unsigned*>(row + ix);
+        else st.r[i] = *reinterpret_cast<const float4*>(row + ix);
       } else {
         const __amdgpu_buffer_rsrc_t rs = ring_rsrc(row, in_row_bytes);
-        float v[V];
+        if constexpr (U8) {
+          unsigned w = 0;
 #pragma unroll
-        for (int e = 0; e < V; ++e)
-          v[e] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, eoff[e], 0, 0));
-        st.r[i] = make_float4(v[0], v[1], v[2], v[3]);
+          for (int e = 0; e < V; ++e)
+            w |= (unsigned)__builtin_amdgcn_raw_buffer_load_b8(rs, eoff[e], 0, 0) << (8 * e);
+          st.r[i] = w;
+        } else {
+          float v[V];
+#pragma unroll
+          for (int e = 0; e < V; ++e)
+            v[e] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, eoff[e], 0, 0));
+          st.r[i] = make_float4(v[0], v[1], v[2], v[3]);
+        }
       }
     }
     return st;
   };
   // horizontal pass of the staged rows, written to LDS slot c % 3
-  auto write_chunk = [&](int c, const RingStage<false, QR>& st) {
+  auto write_chunk = [&](int c, const RingSepStage<TI, QR>& st) {
     const int slot = c % 3;
 #pragma unroll
     for (int i = 0; i < QR; ++i) {
-      float cur[V] = {st.r[i].x, st.r[i].y, st.r[i].z, st.r[i].w};
+      float cur[V];
+      if constexpr (U8) {
+#pragma unroll
+        for (int e = 0; e < V; ++e) cur[e] = (float)((st.r[i] >> (8 * e)) & 0xffu);
+      } else {
+        cur[0] = st.r[i].x;
+        cur[1] = st.r[i].y;
+        cur[2] = st.r[i].z;
+        cur[3] = st.r[i].w;
+      }
       if constexpr (EDGE) {
         if (cmode) {
 #pragma unroll
-          for (int e = 0; e < V; ++e) cur[e] = ecst[e] ? cv : cur[e];
+          for (int e = 0; e < V; ++e) cur[e] = ecst[e] ? cpx : cur[e];
         }
       }
       float win[NWIN];
@@ -501,7 +533,7 @@ __device__ __forceinline__ void ring_sep_body(const SepArgs& a, float4* __restri
       }
       if (cmode && ((st.rc >> i) & 1)) {
 #pragma unroll
-        for (int e = 0; e < V; ++e) h[e] = cv;
+        for (int e = 0; e < V; ++e) h[e] = crow;
       }
       lds[(slot * CH + wave + 4 * i) * 64 + lane] = make_float4(h[0], h[1], h[2], h[3]);
     }
@@ -509,15 +541,15 @@ __device__ __forceinline__ void ring_sep_body(const SepArgs& a, float4* __restri
 
   const bool store_lane = lane >= HL && lane < 64 - HL;
   const bool full = store_lane && (!EDGE || (a.out_vec_ok && xl + V <= g.is_w));
-  const int voff = full ? xl * 4 : kRingDrop;
+  const int voff = full ? xl * BO : kRingDrop;
   int soff[V];
   if constexpr (EDGE) {
 #pragma unroll
     for (int e = 0; e < V; ++e)
-      soff[e] = (store_lane && !full && xl + e < g.is_w) ? (xl + e) * 4 : kRingDrop;
+      soff[e] = (store_lane && !full && xl + e < g.is_w) ? (xl + e) * BO : kRingDrop;
   }
-  float* obase = out + (long long)(g.out_oy + y0) * g.out_stride + g.out_ox;
-  const int out_row_bytes = g.is_w * 4;
+  TO* obase = out + (long long)(g.out_oy + y0) * g.out_stride + g.out_ox;
+  const int out_row_bytes = g.is_w * BO;
 
   auto compute = [&](int s) {
     const int jw = wave * QR;
@@ -545,22 +577,37 @@ __device__ __forceinline__ void ring_sep_body(const SepArgs& a, float4* __restri
       const int j = j0 + qq;
       const __amdgpu_buffer_rsrc_t rs =
           ring_rsrc(obase + (long long)j * g.out_stride, j < nout ? out_row_bytes : 0);
-      const ring_u4 v = {__builtin_bit_cast(unsigned, acc[qq][0]),
-                         __builtin_bit_cast(unsigned, acc[qq][1]),
-                         __builtin_bit_cast(unsigned, acc[qq][2]),
-                         __builtin_bit_cast(unsigned, acc[qq][3])};
-      __builtin_amdgcn_raw_buffer_store_b128(v, rs, voff, 0, 2);
-      if constexpr (EDGE) {
+      if constexpr (std::is_same<TO, float>::value) {
+        const ring_u4 v = {__builtin_bit_cast(unsigned, acc[qq][0]),
+                           __builtin_bit_cast(unsigned, acc[qq][1]),
+                           __builtin_bit_cast(unsigned, acc[qq][2]),
+                           __builtin_bit_cast(unsigned, acc[qq][3])};
+        __builtin_amdgcn_raw_buffer_store_b128(v, rs, voff, 0, 2);
+        if constexpr (EDGE) {
 #pragma unroll
-        for (int e = 0; e < V; ++e) __builtin_amdgcn_raw_buffer_store_b32(v[e], rs, soff[e], 0, 2);
+          for (int e = 0; e < V; ++e) __builtin_amdgcn_raw_buffer_store_b32(v[e], rs, soff[e], 0, 2);
+        }
+      } else {
+        unsigned char px[V];
+        unsigned w = 0;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          px[e] = (unsigned char)cvt_out<TO>::from_f(acc[qq][e] + rnd);
+          w |= (unsigned)px[e] << (8 * e);
+        }
+        __builtin_amdgcn_raw_buffer_store_b32(w, rs, voff, 0, 2);
+        if constexpr (EDGE) {
+#pragma unroll
+          for (int e = 0; e < V; ++e) __builtin_amdgcn_raw_buffer_store_b8(px[e], rs, soff[e], 0, 2);
+        }
       }
     }
   };
 
   const int nos = (nout + CH - 1) / CH;
-  RingStage<false, QR> sets[NS];
+  RingSepStage<TI, QR> sets[NS];
   {
-    const RingStage<false, QR> c0 = load_chunk(0), c1 = load_chunk(1);
+    const RingSepStage<TI, QR> c0 = load_chunk(0), c1 = load_chunk(1);
     write_chunk(0, c0);
     write_chunk(1, c1);
   }
@@ -589,7 +636,7 @@ __device__ __forceinline__ void ring_sep_body(const SepArgs& a, float4* __restri
   }
 }
 
-template <int KX, int KY, bool FMA, int CH, int NS>
+template <typename TI, typename TO, int KX, int KY, bool FMA, int CH, int NS>
 __global__ __launch_bounds__(256) void ring_sep_kernel(const SepArgs a, const RingGeom rg) {
   constexpr int HL = (KX / 2 + kRingV - 1) / kRingV;
   constexpr int SW = (64 - 2 * HL) * kRingV;
@@ -597,8 +644,8 @@ __global__ __launch_bounds__(256) void ring_sep_kernel(const SepArgs a, const Ri
   const Geom& g = a.g;
   if ((int)blockIdx.x >= rg.items) return;
   const RingItem it = ring_item(rg, g, blockIdx.x);
-  const float* in = static_cast<const float*>(g.in) + (long long)it.b * g.in_plane;
-  float* out = static_cast<float*>(g.out) + (long long)it.b * g.out_plane;
+  const TI* in = static_cast<const TI*>(g.in) + (long long)it.b * g.in_plane;
+  TO* out = static_cast<TO*>(g.out) + (long long)it.b * g.out_plane;
   const int lane = threadIdx.x & 63;
   const int xs = it.strip * SW - HL * kRingV;
   int lx, hx, ly, hy;
@@ -607,11 +654,11 @@ __global__ __launch_bounds__(256) void ring_sep_kernel(const SepArgs a, const Ri
   const bool raw = a.in_vec_ok && a.out_vec_ok && ix0 >= lx && ix0 + 64 * kRingV <= hx &&
                    xs + (64 - HL) * kRingV <= g.is_w;
   if (raw)
-    ring_sep_body<KX, KY, FMA, false, CH, NS>(a, lds, in, out, xs + lane * kRingV, it.y0,
-                                             it.nout);
+    ring_sep_body<TI, TO, KX, KY, FMA, false, CH, NS>(a, lds, in, out, xs + lane * kRingV,
+                                                      it.y0, it.nout);
   else
-    ring_sep_body<KX, KY, FMA, true, CH, NS>(a, lds, in, out, xs + lane * kRingV, it.y0,
-                                            it.nout);
+    ring_sep_body<TI, TO, KX, KY, FMA, true, CH, NS>(a, lds, in, out, xs + lane * kRingV,
+                                                     it.y0, it.nout);
 }
 
 }  // namespace hh

@@ -115,7 +115,7 @@ static int launch_ring(const LocalArgs& a0, int seg_rows, hipStream_t s) {
 // resident blocks as fit (the ring takes 48 KB of LDS).
 constexpr int kRingSepChunk = 16;
 
-template <int K, bool FMA>
+template <typename T, int K, bool FMA>
 static int launch_ring_sep(const SepArgs& a0, int seg_rows, hipStream_t s) {
   constexpr int HL = (K / 2 + kRingV - 1) / kRingV;
   static const int slots = [] {
@@ -123,7 +123,8 @@ static int launch_ring_sep(const SepArgs& a0, int seg_rows, hipStream_t s) {
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(&ring_sep_kernel<K, K, FMA, kRingSepChunk, 2>),
+            &per_cu,
+            reinterpret_cast<const void*>(&ring_sep_kernel<T, T, K, K, FMA, kRingSepChunk, 2>),
             256, 0) != hipSuccess)
       return 0;
     return cus * (per_cu < 3 ? per_cu : 3);
@@ -136,22 +137,27 @@ static int launch_ring_sep(const SepArgs& a0, int seg_rows, hipStream_t s) {
                      strips * a.g.is_h * a.g.batch);
   const RingGeom rg = ring_geom(a.g, a.in_vec_ok, a.out_vec_ok, 1.3f, HL, kRingSepChunk,
                                 seg_rows, slots);
-  hipLaunchKernelGGL((ring_sep_kernel<K, K, FMA, kRingSepChunk, 2>), dim3(rg.items), dim3(256),
-                     0, s, a, rg);
+  hipLaunchKernelGGL((ring_sep_kernel<T, T, K, K, FMA, kRingSepChunk, 2>), dim3(rg.items),
+                     dim3(256), 0, s, a, rg);
   return hip_status(hipGetLastError(), "ring_sep_kernel launch");
 }
 
+// f32 -> f32 (no rounding: the float output is the sum) and u8 -> u8
+// (rounding as requested, saturating).
 int dispatch_ring_sep(const SepArgs& a, int acc_is_float, int fma, int seg_rows, hipStream_t s) {
   const Geom& g = a.g;
-  if (!acc_is_float || a.kx != a.ky || g.in_dtype != HIPACC_F32 || g.out_dtype != HIPACC_F32 ||
-      g.post_div != 0.0f || g.cvt == HIPACC_CVT_ROUND)
+  const bool f32 = g.in_dtype == HIPACC_F32 && g.out_dtype == HIPACC_F32;
+  const bool u8 = g.in_dtype == HIPACC_U8 && g.out_dtype == HIPACC_U8;
+  if (!acc_is_float || a.kx != a.ky || !(f32 || u8) || g.post_div != 0.0f ||
+      (f32 && g.cvt == HIPACC_CVT_ROUND))
     return HIPACC_ERR_UNSUPPORTED;
   if ((long long)g.in_imgw * 4 >= (1ll << 31) || (long long)g.is_w * 4 >= (1ll << 31))
     return HIPACC_ERR_UNSUPPORTED;
-#define HH_RSEP(K)                                                                     \
-  if (a.kx == K) return fma ? launch_ring_sep<K, true>(a, seg_rows, s)                 \
-                            : launch_ring_sep<K, false>(a, seg_rows, s);
-  HH_RSEP(3) HH_RSEP(5) HH_RSEP(7) HH_RSEP(9) HH_RSEP(13)
+#define HH_RSEP(T, K)                                                                  \
+  if (a.kx == K) return fma ? launch_ring_sep<T, K, true>(a, seg_rows, s)              \
+                            : launch_ring_sep<T, K, false>(a, seg_rows, s);
+  if (f32) { HH_RSEP(float, 3) HH_RSEP(float, 5) HH_RSEP(float, 7) HH_RSEP(float, 9) HH_RSEP(float, 13) }
+  else { HH_RSEP(uint8_t, 3) HH_RSEP(uint8_t, 5) HH_RSEP(uint8_t, 7) HH_RSEP(uint8_t, 9) HH_RSEP(uint8_t, 13) }
 #undef HH_RSEP
   return HIPACC_ERR_UNSUPPORTED;
 }
