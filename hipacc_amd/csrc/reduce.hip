@@ -228,6 +228,38 @@ __global__ __launch_bounds__(256) void histogram_kernel(const HistArgs a) {
   extern __shared__ uint32_t hist[];
   for (int i = threadIdx.x; i < a.nbins; i += 256) hist[i] = 0u;
   __syncthreads();
+  auto count = [&](const uint4& w) {
+    T t[VEC];
+    __builtin_memcpy(t, &w, 16);
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+      const int b = bin_of<T>(t[k], a.nbins, a.scale);
+      if (b >= 0 && b < a.nbins) atomicAdd(&hist[b], 1u);
+    }
+  };
+  if (a.vec_ok && a.stride == a.w) {
+    // rows back to back: one grid-stride sweep with 4 loads in flight per
+    // thread
+    constexpr int U = 4;
+    const T* base = static_cast<const T*>(a.in) + (long long)a.oy * a.stride + a.ox;
+    const long long nv = (long long)a.h * a.w / VEC;
+    const long long st = (long long)gridDim.x * 256;
+    const uint4* v = reinterpret_cast<const uint4*>(base);
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    for (; i + (U - 1) * st < nv; i += U * st) {
+      uint4 w[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) w[u] = v[i + u * st];
+#pragma unroll
+      for (int u = 0; u < U; ++u) count(w[u]);
+    }
+    for (; i < nv; i += st) count(v[i]);
+    const long long n = (long long)a.h * a.w;
+    for (long long x = nv * VEC + (long long)blockIdx.x * 256 + threadIdx.x; x < n; x += st) {
+      const int b = bin_of<T>(base[x], a.nbins, a.scale);
+      if (b >= 0 && b < a.nbins) atomicAdd(&hist[b], 1u);
+    }
+  } else
   for (int y = blockIdx.x; y < a.h; y += gridDim.x) {
     const T* row = static_cast<const T*>(a.in) + (long long)(a.oy + y) * a.stride + a.ox;
     int start = 0;
@@ -256,7 +288,14 @@ __global__ __launch_bounds__(256) void histogram_kernel(const HistArgs a) {
 }
 
 int launch_histogram(const HistArgs& a, int dt, hipStream_t s) {
-  const int blocks = a.h < kRedBlocks ? a.h : kRedBlocks;
+  const int eb = dt == HIPACC_U8 ? 1 : dt == HIPACC_U16 ? 2 : 4;
+  const long long n = (long long)a.w * a.h;
+  // contiguous windows: 512 blocks (fewer per-block merges into the global
+  // bins); else one block per row, at most 1024
+  const bool flat = a.vec_ok && a.stride == a.w;
+  const long long nflat = (n * eb + 4095) / 4096;
+  const int blocks = flat ? (int)(nflat < 512 ? (nflat > 0 ? nflat : 1) : 512)
+                          : (a.h < kRedBlocks ? a.h : kRedBlocks);
   const size_t smem = (size_t)a.nbins * 4;
   switch (dt) {
     case HIPACC_U8:
