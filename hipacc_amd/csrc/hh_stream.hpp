@@ -38,17 +38,18 @@ struct StreamGeom {
 };
 
 // Cross-lane moves by one lane through DPP wave shifts (gfx9 wave_shr:1 /
-// wave_shl:1).  The lane without a source keeps its own value; the kernels
-// never use what the edge lanes receive.
+// wave_shl:1).  The lane without a source receives 0 (bound_ctrl), which
+// spares the copy of the old value a DPP move with a kept value needs; the
+// kernels never use what the edge lanes receive.
 template <typename T> __device__ __forceinline__ T from_prev_lane(T v) {
   static_assert(sizeof(T) == 4, "32-bit lane values");
   const int x = __builtin_bit_cast(int, v);
-  return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(x, x, 0x138, 0xf, 0xf, false));
+  return __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(x, 0x138, 0xf, 0xf, true));
 }
 template <typename T> __device__ __forceinline__ T from_next_lane(T v) {
   static_assert(sizeof(T) == 4, "32-bit lane values");
   const int x = __builtin_bit_cast(int, v);
-  return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(x, x, 0x130, 0xf, 0xf, false));
+  return __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(x, 0x130, 0xf, 0xf, true));
 }
 template <typename T> __device__ __forceinline__ T lane_shift(T v, int d) {
   // value of lane (l + d); d is a compile-time constant after unrolling
