@@ -93,6 +93,8 @@ int launch_separable(const SepArgs& a, int acc_is_float, int fp_contract,
                      hipStream_t stream, int variant, int param);
 int dispatch_stream_local(const LocalArgs& a, int acc_is_float, int seg, int wide, hipStream_t s);
 int dispatch_ring_local(const LocalArgs& a, int seg_rows, hipStream_t s);
+int dispatch_ring8_local(const LocalArgs& a, int acc_is_float, int seg_rows, hipStream_t s);
+bool ring8_pays(const LocalArgs& a);
 int dispatch_ring_sep(const SepArgs& a, int acc_is_float, int fma, int seg_rows, hipStream_t s);
 int dispatch_stream_sep(const SepArgs& a, int acc_is_float, int fma, int seg, int wide,
                         hipStream_t s);

@@ -41,6 +41,7 @@ constexpr int kRingDrop = (int)0x80000000;      // buffer offset past every rang
 // carries about the same time.
 struct RingGeom {
   int strips, nl, nr;
+  int sw;                    // stored columns per strip
   int segs, seg_rows;        // interior strips
   int segs_e, seg_rows_e;    // edge strips
   int per_img;               // items (strip segments) per image
@@ -48,6 +49,10 @@ struct RingGeom {
 };
 
 typedef unsigned int ring_u4 __attribute__((ext_vector_type(4)));
+
+// Strip classification and segment sizes of a launch (ring_2d.hip).
+RingGeom ring_geom(const Geom& g, int in_vec_ok, int out_vec_ok, float edge_cost, int hl, int v,
+                   bool balance, int chunk, int seg_rows, int slots);
 
 // Buffer resource over `bytes` bytes from `base` (raw buffer, no stride):
 // accesses at offsets past `bytes` are dropped (stores) or read 0 (loads).
@@ -76,16 +81,6 @@ template <int QR> struct RingStage<true, QR> {
 template <int N> __device__ __forceinline__ void ring_wait_vm() {
   static_assert(N >= 0 && N < 64, "vmcnt range");
   asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
-}
-// ring_wait_vm<N> for a count known only after unrolling (waits for at most
-// min(n, N) outstanding operations).
-template <int N> __device__ __forceinline__ void ring_wait_vm_le(int n) {
-  if constexpr (N == 0) {
-    ring_wait_vm<0>();
-  } else {
-    if (n >= N) ring_wait_vm<N>();
-    else ring_wait_vm_le<N - 1>(n);
-  }
 }
 __device__ __forceinline__ void ring_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
@@ -273,10 +268,13 @@ __device__ __forceinline__ void ring_body(const LocalArgs& a, float4* __restrict
   const int nos = (nout + CH - 1) / CH;
   if constexpr (!EDGE) {
     // Direct loads: chunk c lands in slot c % NSLOT.  Step s waits for its
-    // own loads of chunk s+1 (the counter also holds the loads and stores
-    // issued after them), meets the other waves at the barrier, issues the
-    // loads of chunk s+ND+1 into the slot chunk s-1 held, and computes
-    // output chunk s from chunks s and s+1.
+    // own loads of chunk s+1, meets the other waves at the barrier, issues
+    // the loads of chunk s+ND+1 into the slot chunk s-1 held, and computes
+    // output chunk s from chunks s and s+1.  The wait leaves at most the
+    // QR*(ND-1) loads issued after chunk s+1 outstanding: loads retire in
+    // issue order, so while a load of chunk s+1 is outstanding every later
+    // load is too and the count stays above that bound, whatever stores
+    // are pending in the same counter.
     auto issue = [&](int c) {
       const int slot = c % NSLOT;
 #pragma unroll
@@ -292,18 +290,8 @@ __device__ __forceinline__ void ring_body(const LocalArgs& a, float4* __restrict
     };
 #pragma unroll
     for (int c = 0; c <= ND; ++c) issue(c);
-    // first ND steps: chunk s+1 came from the prologue
-#pragma unroll
-    for (int s = 0; s < ND; ++s) {
-      if (s < nos) {
-        ring_wait_vm_le<QR * (2 * ND - 1)>(QR * (ND - 1 + s));
-        ring_barrier();
-        issue(s + ND + 1);
-        compute(s);
-      }
-    }
-    for (int s = ND; s < nos; ++s) {
-      ring_wait_vm<2 * QR * (ND - 1)>();
+    for (int s = 0; s < nos; ++s) {
+      ring_wait_vm<QR * (ND - 1)>();
       ring_barrier();
       issue(s + ND + 1);
       compute(s);

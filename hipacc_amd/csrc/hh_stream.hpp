@@ -415,7 +415,16 @@ static inline StreamGeom stream_geom(const Geom& g, int rx, int v, int seg_rows)
   const int hl = (rx + v - 1) / v;
   sg.strip_w = (64 - 2 * hl) * v;
   sg.strips = ceil_div(g.is_w, sg.strip_w);
-  sg.seg_rows = seg_rows > 0 ? seg_rows : 12;
+  if (seg_rows <= 0) {
+    // 12-row segments, shorter on small images until there are about 2560
+    // waves (10 per CU): a wave walks its rows one after another, so few
+    // long segments leave the launch latency-bound (1024^2 u8: 2-row
+    // segments 8.8 us, 12-row 13 us)
+    const long long wave_rows = (long long)sg.strips * g.is_h * g.batch;
+    seg_rows = (int)((wave_rows + 2559) / 2560);
+    seg_rows = seg_rows < 2 ? 2 : seg_rows > 12 ? 12 : seg_rows;
+  }
+  sg.seg_rows = seg_rows;
   sg.segs = ceil_div(g.is_h, sg.seg_rows);
   sg.total_waves = sg.strips * sg.segs * g.batch;
   sg.xcd_perm = 0;

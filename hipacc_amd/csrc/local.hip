@@ -106,6 +106,7 @@ static int launch_local_variant(const LocalArgs& f, int acc_is_float, hipStream_
     case kVarStreamWide:
       return dispatch_stream_local(f, acc_is_float, param, 1, s);
     case kVarRing:
+      if (f.g.in_dtype == HIPACC_U8) return dispatch_ring8_local(f, acc_is_float, param, s);
       return acc_is_float ? dispatch_ring_local(f, param, s) : HIPACC_ERR_UNSUPPORTED;
     case kVarTile: {
       if (!acc_is_float) return dispatch_sum_int(f, param, s);
@@ -118,7 +119,8 @@ static int launch_local_variant(const LocalArgs& f, int acc_is_float, hipStream_
 }
 
 // variant kVarAuto: the tuned choice if any, else ring -> stream -> tile ->
-// generic.
+// generic (8-bit inputs skip the ring on narrow images, where the streaming
+// kernel is faster).
 // A forced variant returns HIPACC_ERR_UNSUPPORTED when it has no kernel for
 // the operator's shape.
 int launch_local(const LocalArgs& a, int acc_is_float, hipStream_t s, int variant, int param) {
@@ -153,7 +155,9 @@ int launch_local(const LocalArgs& a, int acc_is_float, hipStream_t s, int varian
     const int st = launch_local_variant(f, acc_is_float, s, v.variant, v.param);
     if (st != HIPACC_ERR_UNSUPPORTED) return st;
   }
-  int st = launch_local_variant(f, acc_is_float, s, kVarRing, 0);
+  int st = HIPACC_ERR_UNSUPPORTED;
+  if (a.g.in_dtype != HIPACC_U8 || ring8_pays(f))
+    st = launch_local_variant(f, acc_is_float, s, kVarRing, 0);
   if (st == HIPACC_ERR_UNSUPPORTED) st = launch_local_variant(f, acc_is_float, s, kVarStream, 0);
   if (st == HIPACC_ERR_UNSUPPORTED) st = launch_local_variant(f, acc_is_float, s, kVarTile, 0);
   if (st == HIPACC_ERR_UNSUPPORTED) st = launch_generic(a, acc_is_float, s);

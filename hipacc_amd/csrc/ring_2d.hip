@@ -37,25 +37,30 @@ static int ring_slots() {
 // masks hides most of the extra memory instructions.
 static float ring_edge_cost(int k) { return k <= 5 ? 1.4f : 1.15f; }
 
-// Classifies the strips the way ring_kernel / ring_sep_kernel do (edge: the
-// strip's staged columns leave the raw-readable region, or its stored
-// columns are not all full vectors; hl halo lanes per side) and sizes the
-// segments.  seg_rows > 0 forces the interior segment height; 0 picks it so
-// that one round of items fills `slots`.
-static RingGeom ring_geom(const Geom& g, int in_vec_ok, int out_vec_ok, float edge_cost, int hl,
-                          int chunk, int seg_rows, int slots) {
+// Classifies the strips the way ring_kernel / ring_sep_kernel / ring8_kernel
+// do (edge: the strip's staged columns leave the raw-readable region, or its
+// stored columns are not all full vectors; v pixels per lane, hl halo lanes
+// per side) and sizes the segments.  balance: the strips store the fewest
+// whole lanes that still cover the width in the same number of strips (the
+// lanes past them idle), instead of a narrow last strip.  seg_rows > 0 forces
+// the interior segment height; 0 picks it so that one round of items fills
+// `slots`.
+RingGeom ring_geom(const Geom& g, int in_vec_ok, int out_vec_ok, float edge_cost, int hl, int v,
+                   bool balance, int chunk, int seg_rows, int slots) {
   RingGeom rg;
-  const int sw = (64 - 2 * hl) * kRingV;
-  rg.strips = ceil_div(g.is_w, sw);
+  int sl = 64 - 2 * hl;                 // stored lanes per strip
+  rg.strips = ceil_div(g.is_w, sl * v);
+  if (balance) sl = ceil_div(ceil_div(g.is_w, rg.strips), v);
+  rg.sw = sl * v;
   int lx = g.lo_x, hx = g.hi_x;
   if (g.boundary == HIPACC_BOUNDARY_UNDEFINED) {
     lx = 0;
     hx = g.in_imgw;
   }
   auto edge = [&](int strip) {
-    const int xs = strip * sw - hl * kRingV, ix0 = g.in_ox + xs;
-    return !(in_vec_ok && out_vec_ok && ix0 >= lx && ix0 + 64 * kRingV <= hx &&
-             xs + (64 - hl) * kRingV <= g.is_w);
+    const int xs = strip * rg.sw - hl * v, ix0 = g.in_ox + xs;
+    return !(in_vec_ok && out_vec_ok && ix0 >= lx && ix0 + (sl + 2 * hl) * v <= hx &&
+             xs + (sl + hl) * v <= g.is_w);
   };
   rg.nl = 0;
   while (rg.nl < rg.strips && edge(rg.nl)) ++rg.nl;
@@ -104,8 +109,8 @@ static int launch_ring(const LocalArgs& a0, int seg_rows, hipStream_t s) {
   if (strips * a.g.is_h * a.g.batch > (1ll << 31))
     return set_error(HIPACC_ERR_UNSUPPORTED, "ring kernel: %lld strip rows",
                      strips * a.g.is_h * a.g.batch);
-  const RingGeom rg = ring_geom(a.g, a.in_vec_ok, a.out_vec_ok, ring_edge_cost(K), 1, kRingChunk,
-                                seg_rows, ring_slots<K, FMA, WDEV>());
+  const RingGeom rg = ring_geom(a.g, a.in_vec_ok, a.out_vec_ok, ring_edge_cost(K), 1, kRingV,
+                                false, kRingChunk, seg_rows, ring_slots<K, FMA, WDEV>());
   hipLaunchKernelGGL((ring_kernel<K, FMA, WDEV, kRingChunk, kRingSets, kRingDepth>), dim3(rg.items), dim3(256),
                      0, s, a, rg);
   return hip_status(hipGetLastError(), "ring_kernel launch");
@@ -135,8 +140,8 @@ static int launch_ring_sep(const SepArgs& a0, int seg_rows, hipStream_t s) {
   if (strips * a.g.is_h * a.g.batch > (1ll << 31))
     return set_error(HIPACC_ERR_UNSUPPORTED, "ring_sep kernel: %lld strip rows",
                      strips * a.g.is_h * a.g.batch);
-  const RingGeom rg = ring_geom(a.g, a.in_vec_ok, a.out_vec_ok, 1.3f, HL, kRingSepChunk,
-                                seg_rows, slots);
+  const RingGeom rg = ring_geom(a.g, a.in_vec_ok, a.out_vec_ok, 1.3f, HL, kRingV, false,
+                                kRingSepChunk, seg_rows, slots);
   hipLaunchKernelGGL((ring_sep_kernel<T, T, K, K, FMA, kRingSepChunk, 2>), dim3(rg.items),
                      dim3(256), 0, s, a, rg);
   return hip_status(hipGetLastError(), "ring_sep_kernel launch");
