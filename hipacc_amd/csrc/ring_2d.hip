@@ -9,7 +9,10 @@ namespace hh {
 
 constexpr int kRingChunk = 8;   // rows per LDS chunk
 constexpr int kRingSets = 2;    // register sets of staged rows in flight (edge strips)
-constexpr int kRingDepth = 1;   // chunks of direct loads in flight (interior strips)
+// Chunks of direct loads in flight (interior strips): 2 for 7x7 and 9x9
+// (4096^2: 40 -> 37.5 us and 64 -> 61 us), 1 for 3x3 and 5x5 (5x5 at depth
+// 2: 27.0 vs 26.5 us).
+constexpr int ring_depth(int k) { return k >= 7 ? 2 : 1; }
 
 // Blocks resident on the whole device at once, at most 3 per CU (4 for
 // 7x7): on a CU the oldest block is served first, so each further block
@@ -23,8 +26,9 @@ static int ring_slots() {
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu,
-            reinterpret_cast<const void*>(&ring_kernel<K, FMA, WDEV, kRingChunk, kRingSets, kRingDepth>), 256,
-            0) != hipSuccess)
+            reinterpret_cast<const void*>(
+                &ring_kernel<K, FMA, WDEV, kRingChunk, kRingSets, ring_depth(K)>),
+            256, 0) != hipSuccess)
       return 0;
     const int cap = K == 7 ? 4 : 3;
     return cus * (per_cu < cap ? per_cu : cap);
@@ -111,8 +115,8 @@ static int launch_ring(const LocalArgs& a0, int seg_rows, hipStream_t s) {
                      strips * a.g.is_h * a.g.batch);
   const RingGeom rg = ring_geom(a.g, a.in_vec_ok, a.out_vec_ok, ring_edge_cost(K), 1, kRingV,
                                 false, kRingChunk, seg_rows, ring_slots<K, FMA, WDEV>());
-  hipLaunchKernelGGL((ring_kernel<K, FMA, WDEV, kRingChunk, kRingSets, kRingDepth>), dim3(rg.items), dim3(256),
-                     0, s, a, rg);
+  hipLaunchKernelGGL((ring_kernel<K, FMA, WDEV, kRingChunk, kRingSets, ring_depth(K)>),
+                     dim3(rg.items), dim3(256), 0, s, a, rg);
   return hip_status(hipGetLastError(), "ring_kernel launch");
 }
 

@@ -170,7 +170,7 @@ def test_reduce_window_and_golden(gpu):
 
 # ---- histogram ---------------------------------------------------------------
 @pytest.mark.parametrize("shape", [(1, 1), (29, 37), (513, 1027), (4096, 4096)])
-@pytest.mark.parametrize("nbins", [256, 64, 1000])
+@pytest.mark.parametrize("nbins", [256, 64, 1000, 4096, 1])
 def test_histogram_u8(gpu, shape, nbins):
     src = ob.random_image(shape, np.uint8, shape[1] + nbins)
     bins = torch.zeros(nbins, dtype=torch.int32, device="cuda")
