@@ -415,7 +415,9 @@ template <int QR> struct RingSepStage<uint8_t, QR> {
   int rc;
 };
 
-template <typename TI, typename TO, int KX, int KY, bool FMA, bool EDGE, int CH, int NS>
+// ISUM: integer weights and pixels, summed exactly in float (the host admits
+// only masks whose partial sums stay below 2^24), integer output.
+template <typename TI, typename TO, int KX, int KY, bool FMA, bool ISUM, bool EDGE, int CH, int NS>
 __device__ __forceinline__ void ring_sep_body(const SepArgs& a, float4* __restrict__ lds,
                                               const TI* __restrict__ in, TO* __restrict__ out,
                                               int xl, int y0, int nout) {
@@ -435,7 +437,13 @@ __device__ __forceinline__ void ring_sep_body(const SepArgs& a, float4* __restri
   const int ix = g.in_ox + xl;
   const bool cmode = g.boundary == HIPACC_BOUNDARY_CONSTANT;
   const float cpx = (float)const_pixel<TI>(g);   // a pixel outside the window
-  const float crow = g.const_f;                  // an intermediate row outside the window
+  // an intermediate row outside the window
+  const float crow = ISUM ? (float)g.const_i : g.const_f;
+  float wr[KX], wc[KY];
+#pragma unroll
+  for (int i = 0; i < KX; ++i) wr[i] = ISUM ? (float)a.rowi[i] : a.rowf[i];
+#pragma unroll
+  for (int i = 0; i < KY; ++i) wc[i] = ISUM ? (float)a.coli[i] : a.colf[i];
   const float rnd = g.cvt == HIPACC_CVT_ROUND ? 0.5f : 0.0f;
   int eoff[V];
   bool ecst[V];
@@ -516,7 +524,7 @@ __device__ __forceinline__ void ring_sep_body(const SepArgs& a, float4* __restri
         float s0 = 0.0f;
 #pragma unroll
         for (int dx = 0; dx < KX; ++dx)
-          s0 = dx == 0 ? mul_first(a.rowf[dx], win[e]) : madd<FMA>(a.rowf[dx], win[e + dx], s0);
+          s0 = dx == 0 ? mul_first(wr[dx], win[e]) : madd<FMA>(wr[dx], win[e + dx], s0);
         h[e] = s0;
       }
       if (cmode && ((st.rc >> i) & 1)) {
@@ -554,7 +562,7 @@ __device__ __forceinline__ void ring_sep_body(const SepArgs& a, float4* __restri
       for (int qq = 0; qq < QR; ++qq) {
         const int dyi = t - qq;
         if (dyi < 0 || dyi >= KY) continue;
-        const float w = a.colf[dyi];
+        const float w = wc[dyi];
 #pragma unroll
         for (int e = 0; e < V; ++e)
           acc[qq][e] = dyi == 0 ? mul_first(w, hv[e]) : madd<FMA>(w, hv[e], acc[qq][e]);
@@ -576,17 +584,40 @@ __device__ __forceinline__ void ring_sep_body(const SepArgs& a, float4* __restri
           for (int e = 0; e < V; ++e) __builtin_amdgcn_raw_buffer_store_b32(v[e], rs, soff[e], 0, 2);
         }
       } else {
-        unsigned char px[V];
-        unsigned w = 0;
+        TO px[V];
 #pragma unroll
-        for (int e = 0; e < V; ++e) {
-          px[e] = (unsigned char)cvt_out<TO>::from_f(acc[qq][e] + rnd);
-          w |= (unsigned)px[e] << (8 * e);
-        }
-        __builtin_amdgcn_raw_buffer_store_b32(w, rs, voff, 0, 2);
-        if constexpr (EDGE) {
+        for (int e = 0; e < V; ++e)
+          px[e] = ISUM ? cvt_out<TO>::from_i((int32_t)acc[qq][e])
+                       : cvt_out<TO>::from_f(acc[qq][e] + rnd);
+        if constexpr (BO == 1) {
+          unsigned w = 0;
 #pragma unroll
-          for (int e = 0; e < V; ++e) __builtin_amdgcn_raw_buffer_store_b8(px[e], rs, soff[e], 0, 2);
+          for (int e = 0; e < V; ++e) w |= (unsigned)(uint8_t)px[e] << (8 * e);
+          __builtin_amdgcn_raw_buffer_store_b32(w, rs, voff, 0, 2);
+          if constexpr (EDGE) {
+#pragma unroll
+            for (int e = 0; e < V; ++e)
+              __builtin_amdgcn_raw_buffer_store_b8((unsigned char)px[e], rs, soff[e], 0, 2);
+          }
+        } else if constexpr (BO == 2) {
+          typedef unsigned int ring_u2 __attribute__((ext_vector_type(2)));
+          const ring_u2 v = {(unsigned)(uint16_t)px[0] | (unsigned)(uint16_t)px[1] << 16,
+                             (unsigned)(uint16_t)px[2] | (unsigned)(uint16_t)px[3] << 16};
+          __builtin_amdgcn_raw_buffer_store_b64(v, rs, voff, 0, 2);
+          if constexpr (EDGE) {
+#pragma unroll
+            for (int e = 0; e < V; ++e)
+              __builtin_amdgcn_raw_buffer_store_b16((unsigned short)px[e], rs, soff[e], 0, 2);
+          }
+        } else {
+          const ring_u4 v = {__builtin_bit_cast(unsigned, px[0]), __builtin_bit_cast(unsigned, px[1]),
+                             __builtin_bit_cast(unsigned, px[2]), __builtin_bit_cast(unsigned, px[3])};
+          __builtin_amdgcn_raw_buffer_store_b128(v, rs, voff, 0, 2);
+          if constexpr (EDGE) {
+#pragma unroll
+            for (int e = 0; e < V; ++e)
+              __builtin_amdgcn_raw_buffer_store_b32(v[e], rs, soff[e], 0, 2);
+          }
         }
       }
     }
@@ -624,7 +655,7 @@ __device__ __forceinline__ void ring_sep_body(const SepArgs& a, float4* __restri
   }
 }
 
-template <typename TI, typename TO, int KX, int KY, bool FMA, int CH, int NS>
+template <typename TI, typename TO, int KX, int KY, bool FMA, bool ISUM, int CH, int NS>
 __global__ __launch_bounds__(256) void ring_sep_kernel(const SepArgs a, const RingGeom rg) {
   constexpr int HL = (KX / 2 + kRingV - 1) / kRingV;
   constexpr int SW = (64 - 2 * HL) * kRingV;
@@ -642,11 +673,11 @@ __global__ __launch_bounds__(256) void ring_sep_kernel(const SepArgs a, const Ri
   const bool raw = a.in_vec_ok && a.out_vec_ok && ix0 >= lx && ix0 + 64 * kRingV <= hx &&
                    xs + (64 - HL) * kRingV <= g.is_w;
   if (raw)
-    ring_sep_body<TI, TO, KX, KY, FMA, false, CH, NS>(a, lds, in, out, xs + lane * kRingV,
-                                                      it.y0, it.nout);
+    ring_sep_body<TI, TO, KX, KY, FMA, ISUM, false, CH, NS>(a, lds, in, out, xs + lane * kRingV,
+                                                            it.y0, it.nout);
   else
-    ring_sep_body<TI, TO, KX, KY, FMA, true, CH, NS>(a, lds, in, out, xs + lane * kRingV,
-                                                     it.y0, it.nout);
+    ring_sep_body<TI, TO, KX, KY, FMA, ISUM, true, CH, NS>(a, lds, in, out, xs + lane * kRingV,
+                                                           it.y0, it.nout);
 }
 
 }  // namespace hh

@@ -124,7 +124,7 @@ static int launch_ring(const LocalArgs& a0, int seg_rows, hipStream_t s) {
 // resident blocks as fit (the ring takes 48 KB of LDS).
 constexpr int kRingSepChunk = 16;
 
-template <typename T, int K, bool FMA>
+template <typename TI, typename TO, bool ISUM, int K, bool FMA>
 static int launch_ring_sep(const SepArgs& a0, int seg_rows, hipStream_t s) {
   constexpr int HL = (K / 2 + kRingV - 1) / kRingV;
   static const int slots = [] {
@@ -133,7 +133,8 @@ static int launch_ring_sep(const SepArgs& a0, int seg_rows, hipStream_t s) {
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu,
-            reinterpret_cast<const void*>(&ring_sep_kernel<T, T, K, K, FMA, kRingSepChunk, 2>),
+            reinterpret_cast<const void*>(
+                &ring_sep_kernel<TI, TO, K, K, FMA, ISUM, kRingSepChunk, 2>),
             256, 0) != hipSuccess)
       return 0;
     return cus * (per_cu < 3 ? per_cu : 3);
@@ -146,25 +147,48 @@ static int launch_ring_sep(const SepArgs& a0, int seg_rows, hipStream_t s) {
                      strips * a.g.is_h * a.g.batch);
   const RingGeom rg = ring_geom(a.g, a.in_vec_ok, a.out_vec_ok, 1.3f, HL, kRingV, false,
                                 kRingSepChunk, seg_rows, slots);
-  hipLaunchKernelGGL((ring_sep_kernel<T, T, K, K, FMA, kRingSepChunk, 2>), dim3(rg.items),
-                     dim3(256), 0, s, a, rg);
+  hipLaunchKernelGGL((ring_sep_kernel<TI, TO, K, K, FMA, ISUM, kRingSepChunk, 2>),
+                     dim3(rg.items), dim3(256), 0, s, a, rg);
   return hip_status(hipGetLastError(), "ring_sep_kernel launch");
 }
 
-// f32 -> f32 (no rounding: the float output is the sum) and u8 -> u8
-// (rounding as requested, saturating).
+// f32 -> f32 (no rounding: the float output is the sum), u8 -> u8 (rounding
+// as requested, saturating), and integer sums of u8 pixels into i16 / u32 /
+// i32 (box sums and other integer masks), formed in float where every
+// partial sum stays an exact integer: at most 255 * sum|row| per
+// intermediate pixel (or the CONSTANT value), times sum|col|, below 2^24.
 int dispatch_ring_sep(const SepArgs& a, int acc_is_float, int fma, int seg_rows, hipStream_t s) {
   const Geom& g = a.g;
+  if (a.kx != a.ky || g.post_div != 0.0f) return HIPACC_ERR_UNSUPPORTED;
+  const int bo = dtype_bytes(g.out_dtype);
+  if ((long long)g.in_imgw * 4 >= (1ll << 31) || (long long)g.is_w * bo >= (1ll << 31))
+    return HIPACC_ERR_UNSUPPORTED;
+  if (!acc_is_float) {
+    if (g.in_dtype != HIPACC_U8) return HIPACC_ERR_UNSUPPORTED;
+    double sr = 0.0, sc = 0.0;
+    for (int i = 0; i < a.kx; ++i) sr += std::fabs((double)a.rowi[i]);
+    for (int i = 0; i < a.ky; ++i) sc += std::fabs((double)a.coli[i]);
+    double hmax = 255.0 * sr;
+    if (g.boundary == HIPACC_BOUNDARY_CONSTANT && std::fabs((double)g.const_i) > hmax)
+      hmax = std::fabs((double)g.const_i);
+    if (hmax * sc >= 16777216.0) return HIPACC_ERR_UNSUPPORTED;
+#define HH_RSEPI(TO, DT, K)                                                            \
+  if (g.out_dtype == DT && a.kx == K)                                                  \
+    return launch_ring_sep<uint8_t, TO, true, K, true>(a, seg_rows, s);
+    HH_RSEPI(uint32_t, HIPACC_U32, 3) HH_RSEPI(uint32_t, HIPACC_U32, 5)
+    HH_RSEPI(uint32_t, HIPACC_U32, 7) HH_RSEPI(int32_t, HIPACC_I32, 3)
+    HH_RSEPI(int32_t, HIPACC_I32, 5) HH_RSEPI(int32_t, HIPACC_I32, 7)
+    HH_RSEPI(int16_t, HIPACC_I16, 3) HH_RSEPI(int16_t, HIPACC_I16, 5)
+    HH_RSEPI(int16_t, HIPACC_I16, 7)
+#undef HH_RSEPI
+    return HIPACC_ERR_UNSUPPORTED;
+  }
   const bool f32 = g.in_dtype == HIPACC_F32 && g.out_dtype == HIPACC_F32;
   const bool u8 = g.in_dtype == HIPACC_U8 && g.out_dtype == HIPACC_U8;
-  if (!acc_is_float || a.kx != a.ky || !(f32 || u8) || g.post_div != 0.0f ||
-      (f32 && g.cvt == HIPACC_CVT_ROUND))
-    return HIPACC_ERR_UNSUPPORTED;
-  if ((long long)g.in_imgw * 4 >= (1ll << 31) || (long long)g.is_w * 4 >= (1ll << 31))
-    return HIPACC_ERR_UNSUPPORTED;
+  if (!(f32 || u8) || (f32 && g.cvt == HIPACC_CVT_ROUND)) return HIPACC_ERR_UNSUPPORTED;
 #define HH_RSEP(T, K)                                                                  \
-  if (a.kx == K) return fma ? launch_ring_sep<T, K, true>(a, seg_rows, s)              \
-                            : launch_ring_sep<T, K, false>(a, seg_rows, s);
+  if (a.kx == K) return fma ? launch_ring_sep<T, T, false, K, true>(a, seg_rows, s)    \
+                            : launch_ring_sep<T, T, false, K, false>(a, seg_rows, s);
   if (f32) { HH_RSEP(float, 3) HH_RSEP(float, 5) HH_RSEP(float, 7) HH_RSEP(float, 9) HH_RSEP(float, 13) }
   else { HH_RSEP(uint8_t, 3) HH_RSEP(uint8_t, 5) HH_RSEP(uint8_t, 7) HH_RSEP(uint8_t, 9) HH_RSEP(uint8_t, 13) }
 #undef HH_RSEP

@@ -1,4 +1,5 @@
-"""The 8-bit block row ring (16 pixels per lane, balanced strips): interior
+"""The 8-bit block row ring (16 pixels per lane, balanced strips) and the
+separable ring with integer sums of 8-bit pixels: interior
 strips staged by direct loads, edge strips with per-pixel loads, every
 boundary mode, windows with offsets, pitched and unaligned rows, batches,
 segment heights, float sums and exact integer sums -- bit for bit against
@@ -117,3 +118,52 @@ def test_u8_ring_refuses_inexact_int_masks(gpu):
     assert e.value.status == 2
     call.launch()
     np.testing.assert_array_equal(to_host(dst, np.int32), ref)
+
+
+# ---- separable ring with integer sums ------------------------------------------
+def _ring_sep(src, out_dt, row, col, *, pitch=0, seg=0, **kw):
+    ref = np.zeros(src.shape, out_dt)
+    ob.separable(src, ref, row, col, acc_float=False, dst_dtype=ob.CODES[np.dtype(out_dt)],
+                 **kw)
+    dst = empty_gpu(src.shape, out_dt, pitch=pitch)
+    call = op.prepare_separable(to_gpu(src, pitch=pitch), dst, row, col, acc_float=False,
+                                dst_dtype=ob.CODES[np.dtype(out_dt)], **kw)
+    call.with_variant(capi.VAR_RING, seg).launch()
+    return to_host(dst, out_dt), ref
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("k", [3, 5, 7])
+@pytest.mark.parametrize("out", [np.uint32, np.int32, np.int16])
+def test_u8_box_sep_ring(gpu, mode, k, out):
+    """Box sums (all-ones rows and columns) of u8 pixels, exact in float."""
+    print(f"ring sep box k={k} out={np.dtype(out)} mode={mode}", flush=True)
+    src = ob.random_image((45, 2100), np.uint8, 11 * k)
+    ones = np.ones(k, np.int32)
+    for pitch in (0, 2112):
+        got, ref = _ring_sep(src, out, ones, ones, pitch=pitch, boundary=mode, const_val=77)
+        _cmp(got, ref, mode, k // 2, f"pitch={pitch}")
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_u8_int_sep_ring(gpu, mode):
+    """Mixed-sign integer rows and columns (a separable Sobel and a wider
+    derivative), i16 and i32 outputs; a negative CONSTANT value, which the
+    pixel type wraps."""
+    src = ob.random_image((39, 1700), np.uint8, 23)
+    for row, col, out in (([1, 2, 1], [-1, 0, 1], np.int16),
+                          ([1, 4, 6, 4, 1], [-2, -1, 0, 1, 2], np.int32),
+                          ([3, -7, 11, 5, -2, 9, 1], [1, 1, 2, 3, 2, 1, 1], np.int32)):
+        got, ref = _ring_sep(src, out, np.array(row, np.int32), np.array(col, np.int32),
+                             boundary=mode, const_val=-900)
+        _cmp(got, ref, mode, len(row) // 2, f"row={row}")
+
+
+def test_u8_sep_ring_refuses_inexact_masks(gpu):
+    src = ob.random_image((20, 600), np.uint8, 2)
+    row = np.full(7, 300, np.int32)
+    call = op.prepare_separable(to_gpu(src), empty_gpu(src.shape, np.int32), row, row,
+                                acc_float=False, dst_dtype=capi.I32)
+    with pytest.raises(capi.HipaccError) as e:
+        call.with_variant(capi.VAR_RING, 0).launch()
+    assert e.value.status == 2
