@@ -221,16 +221,36 @@ __device__ __forceinline__ void ring_body(const LocalArgs& a, float4* __restrict
       const int off = jw + t;              // row of chunk s (or of chunk s+1 past CH)
       const int c = off >= CH ? 1 : 0;
       const int slot = (s + c) % NSLOT;
-      const float4 q = lds[(slot * CH + off - c * CH) * 64 + lane];
-      const float cur[V] = {q.x, q.y, q.z, q.w};
+      const float4* rp = lds + (slot * CH + off - c * CH) * 64 + lane;
+      const float4 q = *rp;
       constexpr int NWIN = V + 2 * R;
       float win[NWIN];
+      win[R] = q.x;
+      win[R + 1] = q.y;
+      win[R + 2] = q.z;
+      win[R + 3] = q.w;
+      if constexpr (R == V) {
+        // 9x9: the neighbour lanes' whole words, read from LDS at fixed
+        // offsets of the same address (8 VALU lane shifts fewer per row;
+        // lanes 0 and 63 read a guard word or a neighbouring row, and what
+        // they compute is never stored).  Narrower reads of a neighbour's
+        // word conflict on the LDS banks, so smaller masks shift lanes.
+        const float4 p = rp[-1], n = rp[1];
+        win[0] = p.x;
+        win[1] = p.y;
+        win[2] = p.z;
+        win[3] = p.w;
+        win[R + V] = n.x;
+        win[R + V + 1] = n.y;
+        win[R + V + 2] = n.z;
+        win[R + V + 3] = n.w;
+      } else {
+        const float cur[V] = {q.x, q.y, q.z, q.w};
 #pragma unroll
-      for (int wi = 0; wi < NWIN; ++wi) {
-        const int rel = wi - R;
-        const int d = rel >= 0 ? rel / V : -((-rel + V - 1) / V);
-        const int e = rel - d * V;
-        win[wi] = d == 0 ? cur[e] : lane_shift(cur[e], d);
+        for (int k = 0; k < R; ++k) {
+          win[k] = from_prev_lane(cur[V - R + k]);
+          win[R + V + k] = from_next_lane(cur[k]);
+        }
       }
 #pragma unroll
       for (int qq = 0; qq < QR; ++qq) {
@@ -367,7 +387,9 @@ __device__ __forceinline__ RingItem ring_item(const RingGeom& rg, const Geom& g,
 // One item per block of 4 waves.
 template <int K, bool FMA, bool WDEV, int CH, int NS, int ND>
 __global__ __launch_bounds__(256) void ring_kernel(const LocalArgs a, const RingGeom rg) {
-  __shared__ __align__(16) float4 lds[(ND + 2 > 3 ? ND + 2 : 3) * CH * 64];
+  // the ring with one guard word on each side
+  __shared__ __align__(16) float4 lds_ring[(ND + 2 > 3 ? ND + 2 : 3) * CH * 64 + 2];
+  float4* lds = lds_ring + 1;
   const Geom& g = a.g;
   if ((int)blockIdx.x >= rg.items) return;
   const RingItem it = ring_item(rg, g, blockIdx.x);
