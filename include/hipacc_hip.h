@@ -418,12 +418,15 @@ int hipacc_hip_pyramid_plan_destroy(hipacc_pyramid_plan* plan);
  *             cross-lane halos; param = output rows per wave segment
  *   3 generic one pixel per thread, any shape and reduction mode
  *   4 wide    stream with 8 pixels per lane instead of 4
- *   5 ring    (local f32 -> f32 weighted SUM, square masks 3..9) block row
- *             ring in LDS; param = output rows per block segment, 0 = sized
- *             so that every block is resident at once
+ *   5 ring    block row ring in LDS: local weighted SUM f32 -> f32 (square
+ *             masks 3..9), u8 -> u8 and u8 -> i16/u32/i32 integer sums
+ *             (3x3, 5x5); separable f32, u8, and integer sums of u8 pixels;
+ *             param = output rows per block segment, 0 = sized so that
+ *             every block is resident at once
  * All variants compute identical results.  The plain entry points use the
  * tuned variant of the operator's shape when the tune cache has one, else
- * ring, else stream, else tile, else generic.  The tuner times every candidate on the
+ * ring (for 8-bit local operators only from 2977 columns up), else stream,
+ * else tile, else generic.  The tuner times every candidate on the
  * caller's buffers (the output is overwritten) and records the fastest per
  * (kind, mask size, pixel types, accumulator, boundary, contraction, device
  * weights).  The cache lives in the process and can be saved / loaded.
