@@ -122,28 +122,28 @@ __global__ __launch_bounds__(256) void bilat_grad_kernel(const BilatGradArgs a) 
     }
     __syncthreads();
 
-    float xq[4], yq[4], aq[4];
+    float xq[4], cq[4], aq[4];
     float accc[4] = {0.0f, 0.0f, 0.0f, 0.0f}, accs[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     float accn[4] = {0.0f, 0.0f, 0.0f, 0.0f}, accm[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int o = (tr + R) * LW + tc + RH + j;
       xq[j] = xs[o];
-      yq[j] = ys[o];
+      cq[j] = xs[o] - ys[o];   // X(q) - Y(q)
       aq[j] = as[o];
     }
     // One tap row per iteration (not unrolled): the three row windows stay
     // within the register budget of 4 waves per SIMD.
 #pragma unroll 1
     for (int dy = 0; dy < K; ++dy) {
-      float xr[M], ar[DX ? M : 1], yr[DX ? M : 1];
+      float xr[M], ar[DX ? M : 1], zr[DX ? M : 1];
       const int off = (tr + dy) * LW + tc + (RH - R);
 #pragma unroll
       for (int m = 0; m < M; ++m) {
         xr[m] = xs[off + m];
         if constexpr (DX) {
           ar[m] = as[off + m];
-          yr[m] = ys[off + m];
+          zr[m] = xr[m] - ys[off + m];   // X(p) - Y(p)
         }
       }
 #pragma unroll
@@ -155,21 +155,24 @@ __global__ __launch_bounds__(256) void bilat_grad_kernel(const BilatGradArgs a) 
         for (int j = 0; j < 4; ++j) {
           const float v = xr[j + dx];
           const float d = v - xq[j];
+          const float dd = d * d;
           float sc, sn;   // s of the pair for the centre and the neighbour term
           if constexpr (SYM) {
-            sc = sn = __builtin_amdgcn_exp2f(__builtin_fmaf(k2, d * d, lsp));
+            sc = sn = __builtin_amdgcn_exp2f(__builtin_fmaf(k2, dd, lsp));
           } else {
-            const float e = __builtin_amdgcn_exp2f(k2 * (d * d));
+            const float e = __builtin_amdgcn_exp2f(k2 * dd);
             sc = e * spc;
             sn = e * spn;
           }
-          const float u = (sc * d) * (v - yq[j]);
+          // d (X(p) - Y(q)) = d (d + X(q) - Y(q)), and d (X(q) - Y(p)) =
+          // d ((X(p) - Y(p)) - d): one FMA each from d^2
+          const float u = sc * __builtin_fmaf(d, cq[j], dd);
           accc[j] += u;
           accs[j] = __builtin_fmaf(u, d, accs[j]);
           if constexpr (DX) {
             const float w = ar[j + dx] * sn;
             accn[j] += w;
-            accm[j] = __builtin_fmaf(w * d, xq[j] - yr[j + dx], accm[j]);
+            accm[j] = __builtin_fmaf(w, __builtin_fmaf(d, zr[j + dx], -dd), accm[j]);
           }
         }
       }
