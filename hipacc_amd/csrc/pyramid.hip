@@ -190,7 +190,9 @@ __global__ __launch_bounds__(256) void pyramid_level_kernel(const PyrLevelArgs a
           const int ya = yi - uy0, yb = min(yi + 1, H1 - 1) - uy0;
           const float lf = interp_bilinear(fx, fy, g1s[ya * UW + xa], g1s[ya * UW + xb],
                                            g1s[yb * UW + xa], g1s[yb * UW + xb]);
-          L[(long long)y * ls + x] = gls[(y - ry0) * RW + (x - rx0)] - lf;
+          // L[l] is not read again here: non-temporal stores
+          __builtin_nontemporal_store(gls[(y - ry0) * RW + (x - rx0)] - lf,
+                                      L + (long long)y * ls + x);
         }
       }
     }
