@@ -120,11 +120,12 @@ static int launch_ring(const LocalArgs& a0, int seg_rows, hipStream_t s) {
   return hip_status(hipGetLastError(), "ring_kernel launch");
 }
 
-// Separable masks: 16-row chunks (the vertical radius reaches 8), as many
-// resident blocks as fit (the ring takes 48 KB of LDS), one register set of
-// staged rows in flight (4096^2 13 taps f32: 34.8 us, against 37.5 us with
-// two sets; 7x7 box u8 -> u32: 23.0 against 24.6 us).
-constexpr int kRingSepChunk = 16;
+// Separable masks: 8-row chunks up to 9 taps, 16-row chunks for the larger
+// vertical radii (up to 8), at most 3 resident blocks per CU, one register
+// set of staged rows in flight.  At 4096^2: 13 taps f32 34.3 us (37.5 us
+// with two sets); 5 taps f32 25.5 us and 7x7 box u8 -> u32 20 us (16-row
+// chunks: 27.7 and 23.1 us).
+template <int K> constexpr int sep_chunk() { return K <= 9 ? 8 : 16; }
 constexpr int kRingSepSets = 1;
 
 template <typename TI, typename TO, bool ISUM, int K, bool FMA>
@@ -137,7 +138,7 @@ static int launch_ring_sep(const SepArgs& a0, int seg_rows, hipStream_t s) {
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu,
             reinterpret_cast<const void*>(
-                &ring_sep_kernel<TI, TO, K, K, FMA, ISUM, kRingSepChunk, kRingSepSets>),
+                &ring_sep_kernel<TI, TO, K, K, FMA, ISUM, sep_chunk<K>(), kRingSepSets>),
             256, 0) != hipSuccess)
       return 0;
     return cus * (per_cu < 3 ? per_cu : 3);
@@ -149,8 +150,8 @@ static int launch_ring_sep(const SepArgs& a0, int seg_rows, hipStream_t s) {
     return set_error(HIPACC_ERR_UNSUPPORTED, "ring_sep kernel: %lld strip rows",
                      strips * a.g.is_h * a.g.batch);
   const RingGeom rg = ring_geom(a.g, a.in_vec_ok, a.out_vec_ok, 1.3f, HL, kRingV, false,
-                                kRingSepChunk, seg_rows, slots);
-  hipLaunchKernelGGL((ring_sep_kernel<TI, TO, K, K, FMA, ISUM, kRingSepChunk, kRingSepSets>),
+                                sep_chunk<K>(), seg_rows, slots);
+  hipLaunchKernelGGL((ring_sep_kernel<TI, TO, K, K, FMA, ISUM, sep_chunk<K>(), kRingSepSets>),
                      dim3(rg.items), dim3(256), 0, s, a, rg);
   return hip_status(hipGetLastError(), "ring_sep_kernel launch");
 }
