@@ -66,9 +66,6 @@ def time_calls(calls, steps, warmup):
     return e0.elapsed_time(e1) / steps
 
 
-HEAD_CANDIDATES = ("ring:0", "stream:12", "stream:16", "wide:16", "tile:4")
-
-
 def parse_variant(text):
     from hipacc_amd import capi
     name, _, param = text.partition(":")
@@ -76,21 +73,52 @@ def parse_variant(text):
     return codes[name], int(param or 0)
 
 
-def choose_variant(calls, steps=60):
-    """Times each candidate variant on the rotating buffers; returns
-    (ms, name, variant calls) of the fastest."""
+def tuned(calls, iters=60):
+    """The library's autotuner (hipacc_hip_tune_*) on the first call: it
+    times every kernel variant back to back over its own 512 MiB of rotating
+    buffers and records the fastest for this operator and size class, which
+    the calls' automatic choice then runs.  Returns the variant as name:param."""
     from hipacc_amd import capi
-    best = None
-    for name in HEAD_CANDIDATES:
-        v, p = parse_variant(name)
-        vc = [c.with_variant(v, p) for c in calls]
-        try:
-            ms = min(time_calls(vc, steps, 5) for _ in range(2))
-        except capi.HipaccError:
-            continue
-        if best is None or ms < best[0]:
-            best = (ms, name, vc)
-    return best
+    e = calls[0].tune(iters)
+    return f"{capi.VARIANT_NAMES[e.variant]}:{e.param}"
+
+
+def median_ms(calls, steps, warmup, batches=5):
+    """Median over `batches` timed batches of `steps` launches (the first
+    batch after `warmup` untimed launches, the others after 2)."""
+    samples = [time_calls(calls, steps, warmup if k == 0 else 2) for k in range(batches)]
+    return statistics.median(samples), samples
+
+
+SCIPY_MODE = {0: "nearest", 1: "nearest", 2: "wrap", 3: "reflect", 4: "constant"}
+
+
+def fp64_error(src, dst, *, boundary, weights2d=None, k=0, row=None, col=None, rows=16, y0=2040):
+    """Max |err| of a float result against FP64, on `rows` image rows at
+    full width (the left and right borders included), evaluated from those
+    rows plus the vertical halo: the C oracle's FP64 path (oracle_local_f64)
+    for 2D masks, scipy's float64 correlate1d along x then y for separable
+    ones.  Also relative to max |reference|."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_binding as ob
+    import scipy.ndimage as ndi
+    r = (k if weights2d is not None else len(col)) // 2
+    crop = np.ascontiguousarray(src[y0 - r:y0 + rows + r].cpu().numpy())
+    if weights2d is not None:
+        f64 = np.zeros(crop.shape, np.float64)
+        ob.local(crop, np.zeros_like(crop), k, k, weights=weights2d, boundary=boundary,
+                 f64_out=f64)
+    else:
+        m = SCIPY_MODE[boundary]
+        f64 = ndi.correlate1d(crop.astype(np.float64), np.asarray(row, np.float64), axis=1,
+                              mode=m, cval=0.0)
+        f64 = ndi.correlate1d(f64, np.asarray(col, np.float64), axis=0, mode=m, cval=0.0)
+    ref = f64[r:r + rows]
+    got = dst[y0:y0 + rows].cpu().numpy().astype(np.float64)
+    err = float(np.max(np.abs(got - ref)))
+    return {"max_abs_err_vs_fp64": err,
+            "max_err_rel_to_max_vs_fp64": err / max(float(np.max(np.abs(ref))), 1e-30),
+            "err_rows": [y0, y0 + rows]}
 
 
 def op_entry(name, ms, bytes_per_px, npix=NPIX, extra=None):
@@ -183,14 +211,21 @@ def bench_config4(op, capi, dev, steps):
                 for c in self.calls:
                     c.launch()
 
-        fwd = time_calls([Seq([conv, bil])], steps, 3)
-        fb = time_calls([Seq([conv, bil_t, bil_b, conv_b])], steps, 3)
-        parts = {k: round(time_calls([c], steps, 3), 5)
+        # timed as the headline: warm-up, then the median of 5 batches
+        fwd, fwd_s = median_ms([Seq([conv, bil])], steps, 5)
+        fb, fb_s = median_ms([Seq([conv, bil_t, bil_b, conv_b])], steps, 5)
+        parts = {k: round(median_ms([c], steps, 5)[0], 5)
                  for k, c in (("conv_fwd", conv), ("bilateral_fwd", bil),
                               ("bilateral_train_fwd", bil_t), ("bilateral_bwd", bil_b),
                               ("conv_bwd", conv_b))}
+        fwd_parts = parts["conv_fwd"] + parts["bilateral_fwd"]
+        fb_parts = (parts["conv_fwd"] + parts["bilateral_train_fwd"] + parts["bilateral_bwd"] +
+                    parts["conv_bwd"])
         out[name] = {"fwd_ms": round(fwd, 5), "fwd_bwd_ms": round(fb, 5),
-                     "ratio": round(fb / fwd, 3), "parts_ms": parts,
+                     "ratio": round(fb / fwd, 3),
+                     "fwd_batches_ms": [round(v, 5) for v in fwd_s],
+                     "fwd_bwd_batches_ms": [round(v, 5) for v in fb_s],
+                     "part_sum_ratio": round(fb_parts / fwd_parts, 3), "parts_ms": parts,
                      "conv_ratio": round((parts["conv_fwd"] + parts["conv_bwd"]) /
                                          parts["conv_fwd"], 3),
                      "bilateral_ratio": round((parts["bilateral_train_fwd"] +
@@ -200,6 +235,13 @@ def bench_config4(op, capi, dev, steps):
             "modes": out, "mpix_per_step": B * h * w / 1e6}
 
 
+def pyramid_workload_shapes(n: int = 64, seed: int = 5005):
+    """(H, W) of the batched-pyramid workload: W ~ U{256..4096},
+    H ~ U{256..2160}, seeded."""
+    rng = np.random.default_rng(seed)
+    return [(int(rng.integers(256, 2161)), int(rng.integers(256, 4097))) for _ in range(n)]
+
+
 def bench_config5(op, capi, dev, reps):
     """64 float images, W ~ U{256..4096}, H ~ U{256..2160} (seeded), 8-level
     Gaussian / Laplacian pyramids (5x5 binomial, CLAMP): the batched plan
@@ -207,9 +249,8 @@ def bench_config5(op, capi, dev, reps):
     against the pyramid sample's per-image, per-level launches (convolution,
     NN subsample, LF difference), issued eagerly and also captured in a graph."""
     import torch
-    rng = np.random.default_rng(5005)
-    n, levels = 64, 8
-    shapes = [(int(rng.integers(256, 2161)), int(rng.integers(256, 4097))) for _ in range(n)]
+    levels = 8
+    shapes = pyramid_workload_shapes()
     gauss, lap = [], []
     for i, (h, w) in enumerate(shapes):
         g0 = torch.empty((h, w), dtype=torch.float32, device=dev)
@@ -248,6 +289,18 @@ def bench_config5(op, capi, dev, reps):
             for c in calls:
                 c.launch()
 
+    # the batched plan and the per-level sequence give the same bits (once,
+    # before timing)
+    plan.run()
+    torch.cuda.synchronize()
+    snap = [[t.clone() for t in gauss[i][1:] + lap[i]] for i in range(len(shapes))]
+    Seq().launch()
+    torch.cuda.synchronize()
+    for i in range(len(shapes)):
+        for a, b in zip(snap[i], gauss[i][1:] + lap[i]):
+            if not torch.equal(a, b):
+                raise RuntimeError(f"batched pyramid differs from the per-level launches (image {i})")
+    del snap
     ms_eager = time_calls([Seq()], max(3, reps // 4), 1)
     graph = torch.cuda.CUDAGraph()
     with torch.cuda.graph(graph):
@@ -256,7 +309,8 @@ def bench_config5(op, capi, dev, reps):
     plan.close()
     gbs = 9 * px_levels / (ms_graph * 1e-3) / 1e9
     return {"workload": "64 f32 images W~U{256..4096} H~U{256..2160}, 8 levels, 5x5 binomial, CLAMP",
-            "tile_rows": tile_rows, "batched_graph_ms": round(ms_graph, 4),
+            "tile_rows": tile_rows, "batched_equals_per_level": True,
+            "batched_graph_ms": round(ms_graph, 4),
             "batched_direct_ms": round(ms_direct, 4),
             "per_level_eager_ms": round(ms_eager, 4),
             "per_level_graph_ms": round(ms_eager_graph, 4),
@@ -268,28 +322,41 @@ def bench_config5(op, capi, dev, reps):
             "gbs_at_9B_per_level_px": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(budget_s: float):
-    """The C oracle (OpenMP over rows, per-pixel boundary handling) on whole
-    4096x4096 images of the headline operator, repeated until `budget_s`."""
+    """The C oracle (OpenMP over rows, per-pixel boundary handling, built
+    with gcc -O3) on whole 4096x4096 images of the headline operator,
+    repeated until `budget_s`.  Arithmetic as the reference's generated CPU
+    backend: a rounded product, then a rounded add per tap (no contraction)."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_binding as ob
     from hipacc_amd import capi, masks
     src = ob.random_image((H, W), np.float32, 7)
     dst = np.zeros_like(src)
     w = masks.gaussian_2d(5, 1.0)
-    ob.local(src, dst, 5, 5, weights=w, boundary=capi.CLAMP, fp_contract=True)  # warm-up
+    ob.local(src, dst, 5, 5, weights=w, boundary=capi.CLAMP, fp_contract=False)  # warm-up
     n, t0 = 0, time.perf_counter()
     while True:
-        ob.local(src, dst, 5, 5, weights=w, boundary=capi.CLAMP, fp_contract=True)
+        ob.local(src, dst, 5, 5, weights=w, boundary=capi.CLAMP, fp_contract=False)
         n += 1
         el = time.perf_counter() - t0
         if el >= budget_s:
             break
     threads = ob.lib().oracle_num_threads()
     return {"value": round(n * NPIX / el / 1e6, 2), "unit": "Mpix/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "cpu_model": cpu_model(), "nproc": os.cpu_count(),
             "sample": f"{n} full 4096x4096 float 5x5 CLAMP images ({el:.1f} s), "
-                      f"oracle/hipacc_oracle.c with {threads} OpenMP threads"}
+                      f"oracle/hipacc_oracle.c (gcc -O3 -fopenmp, no contraction: mul then "
+                      f"add per tap) with {threads} OpenMP threads"}
 
 
 def main() -> int:
@@ -337,10 +404,11 @@ def main() -> int:
         torch.cuda.synchronize()
         print(f"profile run: 20 x {args.variant} 5x5 f32 4096^2, 20 x 64 MiB copy", flush=True)
         return 0
-    _, vname, head = choose_variant(head)
-    samples = [time_calls(head, args.steps, args.warmup if k == 0 else 2) for k in range(5)]
-    ms = statistics.median(samples)
-    e = op_entry("conv5x5_f32_clamp", ms, 8)
+    vname = tuned(head)
+    ms, samples = median_ms(head, args.steps, args.warmup)
+    e = op_entry("conv5x5_f32_clamp", ms, 8, extra={"variant": vname})
+    torch.cuda.synchronize()
+    e.update(fp64_error(f32_in[0], f32_out[0], boundary=capi.CLAMP, weights2d=g5, k=5))
     result = {
         "metric": "Mpix/s + HBM GB/s (% peak), 4096^2 stencils; fwd+bwd vs plain fwd time",
         "value": e["mpix_s"],
@@ -391,8 +459,15 @@ def main() -> int:
         g13 = masks.gaussian_1d(13, 3.0)
         ones7 = np.ones(7, np.int32)
 
-        def run(name, calls, bpp, n=steps, extra=None, npix=NPIX):
-            ops[name] = op_entry(name, time_calls(calls, n, 5), bpp, npix=npix, extra=extra)
+        def run(name, calls, bpp, n=steps, extra=None, npix=NPIX, tune=True, err=None):
+            extra = dict(extra or {})
+            if tune:
+                extra["variant"] = tuned(calls)
+            ms, _ = median_ms(calls, n, 5, batches=3)
+            ops[name] = op_entry(name, ms, bpp, npix=npix, extra=extra)
+            if err is not None:
+                torch.cuda.synchronize()
+                ops[name].update(err())
 
         # 1024x1024 uint8 5x5 Gaussian, rounded uint8 output (cache-resident:
         # reported, no roofline claim)
@@ -405,14 +480,17 @@ def main() -> int:
                               cvt=capi.CVT_ROUND) for i in range(NSETS)], 2, npix=1024 * 1024)
 
         run("sobel3x3_f32_clamp", [op.prepare_local(f32_in[i], f32_out[i], 3, 3, weights=sob,
-                                                    boundary=capi.CLAMP) for i in range(NSETS)], 8)
+                                                    boundary=capi.CLAMP) for i in range(NSETS)], 8,
+            err=lambda: fp64_error(f32_in[0], f32_out[0], boundary=capi.CLAMP,
+                                   weights2d=sob.astype(np.float32), k=3))
         run("sobel3x3_u8_i16_clamp", [op.prepare_local(u8_in[i], i16_out[i], 3, 3, weights=sob,
                                                        domain=dom, acc_float=False,
                                                        boundary=capi.CLAMP)
                                       for i in range(NSETS)], 3)
         run("gauss13_sep_f32_clamp", [op.prepare_separable(f32_in[i], f32_out[i], g13, g13,
                                                            boundary=capi.CLAMP)
-                                      for i in range(NSETS)], 8)
+                                      for i in range(NSETS)], 8,
+            err=lambda: fp64_error(f32_in[0], f32_out[0], boundary=capi.CLAMP, row=g13, col=g13))
         run("gauss13_sep_u8_clamp", [op.prepare_separable(u8_in[i], u8_out[i], g13, g13,
                                                           boundary=capi.CLAMP, cvt=capi.CVT_ROUND)
                                      for i in range(NSETS)], 2)
@@ -423,16 +501,26 @@ def main() -> int:
         for mode, mname in ((capi.MIRROR, "mirror"), (capi.REPEAT, "repeat"),
                             (capi.CONSTANT, "constant"), (capi.UNDEFINED, "undefined")):
             run(f"conv5x5_f32_{mname}", [op.prepare_local(f32_in[i], f32_out[i], 5, 5, weights=g5,
-                                                          boundary=mode) for i in range(NSETS)], 8)
+                                                          boundary=mode) for i in range(NSETS)], 8,
+                err=lambda mode=mode: fp64_error(f32_in[0], f32_out[0], boundary=mode,
+                                                 weights2d=g5, k=5))
         res_f = torch.zeros(4, dtype=torch.float32, device=dev)
         res_i = torch.zeros(4, dtype=torch.int32, device=dev)
+        def sum_err():
+            sys.path.insert(0, str(ROOT / "tests"))
+            import oracle_binding as ob
+            exact = ob.sum_f64(f32_in[0].cpu().numpy())
+            got = float(res_f[0].item())
+            return {"abs_err_vs_fp64": abs(got - exact), "rel_err_vs_fp64": abs(got - exact) / exact}
         for mode, mname in ((capi.SUM, "sum"), (capi.MIN, "min"), (capi.MAX, "max")):
             run(f"reduce_{mname}_f32", [op.ReduceCall(f32_in[i], mode, res_f[i:i + 1])
-                                        for i in range(NSETS)], 4)
+                                        for i in range(NSETS)], 4, tune=False,
+                err=sum_err if mode == capi.SUM else None)
         run("reduce_sum_u8_u32", [op.ReduceCall(u8_in[i], capi.SUM, res_i[i:i + 1])
-                                  for i in range(NSETS)], 1)
+                                  for i in range(NSETS)], 1, tune=False)
         bins = torch.zeros((NSETS, 256), dtype=torch.int32, device=dev)
-        run("histogram256_u8", [op.HistogramCall(u8_in[i], 256, bins[i]) for i in range(NSETS)], 1)
+        run("histogram256_u8", [op.HistogramCall(u8_in[i], 256, bins[i]) for i in range(NSETS)], 1,
+            tune=False)
         ops["bilateral13_f32_mirror"] = bench_config3(op, capi, dev, 12)
         result["fwd_bwd"] = bench_config4(op, capi, dev, 20)
         result["pyramid"] = bench_config5(op, capi, dev, 20)

@@ -102,7 +102,8 @@ class PyramidDesc(ctypes.Structure):
 class TuneEntry(ctypes.Structure):
     _fields_ = [("kind", c_int32), ("kx", c_int32), ("ky", c_int32), ("in_dtype", c_int32),
                 ("out_dtype", c_int32), ("acc_dtype", c_int32), ("boundary", c_int32),
-                ("fp_contract", c_int32), ("weights_dev", c_int32), ("variant", c_int32),
+                ("fp_contract", c_int32), ("weights_dev", c_int32), ("size_class", c_int32),
+                ("variant", c_int32),
                 ("param", c_int32), ("ms", c_float)]
 
 
@@ -133,6 +134,8 @@ SIGNATURES = {
     "hipacc_hip_pyramid_run": (c_int32, [c_void_p, c_void_p]),
     "hipacc_hip_pyramid_run_direct": (c_int32, [c_void_p, c_void_p]),
     "hipacc_hip_pyramid_plan_destroy": (c_int32, [c_void_p]),
+    "hipacc_hip_pyramid_plan_tiles": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_int32,
+                                                c_void_p, c_void_p]),
     "hipacc_hip_local_grad": (c_int32, [POINTER(LocalGradDesc), c_void_p]),
     "hipacc_hip_local_grad_variant": (c_int32, [POINTER(LocalGradDesc), c_int32, c_void_p]),
     "hipacc_hip_bilateral_train": (c_int32, [POINTER(BilateralGradDesc), c_void_p]),

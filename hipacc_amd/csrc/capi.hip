@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -53,6 +54,7 @@ struct DeviceState {
 };
 static DeviceState g_dev[kMaxDevices];
 
+
 static int current_device(int* dev) {
   hipError_t e = hipGetDevice(dev);
   if (e != hipSuccess) return hip_status(e, "hipGetDevice");
@@ -72,6 +74,32 @@ static int ensure_workspace(int dev, void** ws) {
     if (e != hipSuccess) return hip_status(e, "hipDeviceSynchronize");
   }
   *ws = d.red_ws;
+  return HIPACC_OK;
+}
+
+// Per-stream workspaces (the histogram's partial bins): the calls on one
+// stream run in order, so a stream's buffer is never used by two calls at
+// once, and calls on different streams (or host threads with their own
+// streams) get different buffers.  Fixed size, never freed before the
+// process ends.
+constexpr size_t kStreamWorkspace = sizeof(uint32_t) * kHistPartialEntries;
+static std::mutex g_stream_ws_mu;
+static std::map<std::pair<int, hipStream_t>, void*> g_stream_ws;
+
+static int stream_workspace(hipStream_t s, void** ws) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_status(e, "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_stream_ws_mu);
+  void*& p = g_stream_ws[std::make_pair(dev, s)];
+  if (!p) {
+    e = hipMalloc(&p, kStreamWorkspace);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return hip_status(e, "hipMalloc(stream workspace)");
+    }
+  }
+  *ws = p;
   return HIPACC_OK;
 }
 
@@ -102,14 +130,18 @@ static int ensure_grad_workspace(size_t bytes, void** ws) {
 // ---------------------------------------------------------------------------
 // Timing (hipacc_last_kernel_timing)
 // ---------------------------------------------------------------------------
-static bool g_timing = false;
-static float g_last_ms = 0.0f;
+// Per host thread, as the reference's timing singleton
+// (runtime/hipacc_base_standalone.hpp:35-39): a thread that enables timing
+// times its own launches and reads its own last time.
+static thread_local bool g_timing = false;
+static thread_local float g_last_ms = 0.0f;
 
 struct Timer {
   hipStream_t s;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  explicit Timer(hipStream_t st) : s(st) {
-    if (g_timing && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess)
+  explicit Timer(hipStream_t st, bool enabled = true) : s(st) {
+    if (enabled && g_timing && hipEventCreate(&e0) == hipSuccess &&
+        hipEventCreate(&e1) == hipSuccess)
       hipEventRecord(e0, s);
   }
   int finish(int st) {
@@ -424,11 +456,60 @@ static int run_separable(const hipacc_separable_desc* d, int variant, int param,
   return tm.finish(launch_separable(a, accf, d->fp_contract != 0, s, variant, param));
 }
 
-// Times every variant/parameter candidate of one operator on the caller's
-// buffers and records the fastest in the tune cache.
+// Buffer sets the tuner rotates over, so that each timed launch reads its
+// input from HBM as it would in a pipeline over large images: copies of the
+// caller's input / output extents (same pitch, same offset within 256 bytes,
+// contents irrelevant to the timing) whose total reaches 512 MiB, twice the
+// Infinity Cache.  Below 8 MiB per set the operator's working set stays in
+// cache in real use as well, and the caller's buffers are used.
+struct TuneRotation {
+  char* base = nullptr;
+  size_t in_bytes = 0, set = 0;
+  uintptr_t in_off = 0, out_off = 0;
+  int n = 0;
+  ~TuneRotation() {
+    if (base) (void)hipFree(base);
+  }
+  static size_t extent(const hipacc_view& v) {
+    const long long nb = v.batch > 1 ? v.batch : 1;
+    return (size_t)dtype_bytes(v.dtype) *
+           ((size_t)v.plane_stride * (size_t)(nb - 1) + (size_t)v.stride * (size_t)v.img_height);
+  }
+  int init(const hipacc_view& in, const hipacc_view& out) {
+    const size_t al = 256;
+    in_bytes = (extent(in) + 2 * al - 1) / al * al;
+    const size_t ob = (extent(out) + 2 * al - 1) / al * al;
+    set = in_bytes + ob;
+    if (set * 64 < ((size_t)512 << 20)) return HIPACC_OK;
+    n = (int)((((size_t)512 << 20) + set - 1) / set);
+    if (n < 2) n = 2;
+    in_off = reinterpret_cast<uintptr_t>(in.data) % al;
+    out_off = reinterpret_cast<uintptr_t>(out.data) % al;
+    const hipError_t e = hipMalloc(&base, set * (size_t)n);
+    if (e != hipSuccess) {
+      base = nullptr;
+      n = 0;
+      return hip_status(e, "hipMalloc(tuner rotation buffers)");
+    }
+    return HIPACC_OK;
+  }
+  // Geometry of launch k: set k % n, or the caller's buffers without sets.
+  void apply(Geom& g, int k) const {
+    if (n == 0 || k < 0) return;
+    char* p = base + (size_t)(k % n) * set;
+    g.in = p + in_off;
+    g.out = p + in_bytes + out_off;
+  }
+};
+
+// Times every variant/parameter candidate of one operator and records the
+// fastest in the tune cache.  run(variant, param, k) launches the candidate
+// on rotation set k (k < 0: the caller's buffers).  Each candidate runs
+// once on the caller's buffers (unsupported ones are skipped), once on every
+// set, then `iters` timed launches back to back over the sets.
 template <typename RunFn>
 static int tune_candidates(RunFn run, const TuneKey& key, int iters, hipStream_t s,
-                           hipacc_tune_entry* best) {
+                           hipacc_tune_entry* best, int nsets) {
   static const int cands[][2] = {
       {kVarStream, 8},      {kVarStream, 16},     {kVarStream, 24},     {kVarStream, 32},
       {kVarStream, 64},     {kVarStreamWide, 8},  {kVarStreamWide, 16}, {kVarStreamWide, 24},
@@ -446,14 +527,16 @@ static int tune_candidates(RunFn run, const TuneKey& key, int iters, hipStream_t
   TuneVal bestv{0, 0, 0.0f};
   int st = HIPACC_OK;
   for (const auto& c : cands) {
-    st = run(c[0], c[1]);
+    st = run(c[0], c[1], -1);
     if (st == HIPACC_ERR_UNSUPPORTED) {
       st = HIPACC_OK;
       continue;
     }
     if (st) break;
+    for (int k = 0; k < nsets && st == HIPACC_OK; ++k) st = run(c[0], c[1], k);
+    if (st) break;
     hipEventRecord(e0, s);
-    for (int i = 0; i < iters && st == HIPACC_OK; ++i) st = run(c[0], c[1]);
+    for (int i = 0; i < iters && st == HIPACC_OK; ++i) st = run(c[0], c[1], nsets ? i % nsets : -1);
     hipEventRecord(e1, s);
     if (st) break;
     if ((e = hipEventSynchronize(e1)) != hipSuccess) {
@@ -465,6 +548,10 @@ static int tune_candidates(RunFn run, const TuneKey& key, int iters, hipStream_t
     ms /= iters;
     if (bestv.variant == 0 || ms < bestv.ms) bestv = TuneVal{c[0], c[1], ms};
   }
+  // the rotation buffers are freed by the caller after this point: the
+  // stream must be idle
+  if (st == HIPACC_OK && (e = hipStreamSynchronize(s)) != hipSuccess)
+    st = hip_status(e, "hipStreamSynchronize");
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (st) return st;
@@ -480,6 +567,7 @@ static int tune_candidates(RunFn run, const TuneKey& key, int iters, hipStream_t
     best->boundary = key.boundary;
     best->fp_contract = key.fma;
     best->weights_dev = key.wdev;
+    best->size_class = key.sz;
     best->variant = bestv.variant;
     best->param = bestv.param;
     best->ms = bestv.ms;
@@ -521,15 +609,28 @@ int hipacc_hip_tune_local(const hipacc_local_desc* d, int32_t iters, hipacc_tune
   int st = prep_local(d, a, accf, box, sa);
   if (st) return st;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  TuneRotation rot;
+  if ((st = rot.init(d->in, d->out))) return st;
+  const int sz = tune_size_class(a.g.is_w, a.g.is_h, a.g.batch);
   if (box) {
-    TuneKey k{kTuneBox, sa.kx, sa.ky, sa.g.in_dtype, sa.g.out_dtype, 0, sa.g.boundary, 0, 0};
-    return tune_candidates([&](int v, int p) { return launch_separable(sa, 0, 0, s, v, p); }, k,
-                           iters, s, best);
+    TuneKey k{kTuneBox, sa.kx, sa.ky, sa.g.in_dtype, sa.g.out_dtype, 0, sa.g.boundary, 0, 0, sz};
+    return tune_candidates(
+        [&](int v, int p, int r) {
+          SepArgs b = sa;
+          rot.apply(b.g, r);
+          return launch_separable(b, 0, 0, s, v, p);
+        },
+        k, iters, s, best, rot.n);
   }
   TuneKey k{kTuneLocal, a.mask_w, a.mask_h, a.g.in_dtype, a.g.out_dtype, accf, a.g.boundary,
-            a.fma, a.wdev != nullptr};
-  return tune_candidates([&](int v, int p) { return launch_local(a, accf, s, v, p); }, k, iters, s,
-                         best);
+            a.fma, a.wdev != nullptr, sz};
+  return tune_candidates(
+      [&](int v, int p, int r) {
+        LocalArgs b = a;
+        rot.apply(b.g, r);
+        return launch_local(b, accf, s, v, p);
+      },
+      k, iters, s, best, rot.n);
 }
 
 int hipacc_hip_tune_separable(const hipacc_separable_desc* d, int32_t iters, hipacc_tune_entry* best,
@@ -540,10 +641,17 @@ int hipacc_hip_tune_separable(const hipacc_separable_desc* d, int32_t iters, hip
   if (st) return st;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int fma = d->fp_contract != 0;
+  TuneRotation rot;
+  if ((st = rot.init(d->in, d->out))) return st;
   TuneKey k{a.box ? kTuneBox : kTuneSeparable, a.kx, a.ky, a.g.in_dtype, a.g.out_dtype, accf,
-            a.g.boundary, fma, 0};
-  return tune_candidates([&](int v, int p) { return launch_separable(a, accf, fma, s, v, p); }, k,
-                         iters, s, best);
+            a.g.boundary, fma, 0, tune_size_class(a.g.is_w, a.g.is_h, a.g.batch)};
+  return tune_candidates(
+      [&](int v, int p, int r) {
+        SepArgs b = a;
+        rot.apply(b.g, r);
+        return launch_separable(b, accf, fma, s, v, p);
+      },
+      k, iters, s, best, rot.n);
 }
 
 int hipacc_hip_tune_set(const hipacc_tune_entry* e) {
@@ -551,7 +659,7 @@ int hipacc_hip_tune_set(const hipacc_tune_entry* e) {
   if (e->variant < kVarTile || e->variant > kVarRing)
     return set_error(HIPACC_ERR_INVALID_ARG, "tune_set: variant %d", e->variant);
   TuneKey k{e->kind, e->kx, e->ky, e->in_dtype, e->out_dtype, e->acc_dtype == HIPACC_F32,
-            e->boundary, e->fp_contract, e->weights_dev};
+            e->boundary, e->fp_contract, e->weights_dev, e->size_class};
   tune_store(k, TuneVal{e->variant, e->param, e->ms});
   return HIPACC_OK;
 }
@@ -657,9 +765,12 @@ int hipacc_hip_reduce_sync(const hipacc_view* img, int32_t reduce, int32_t acc_d
   if ((st = current_device(&dev))) return st;
   void* ws = nullptr;
   if ((st = ensure_workspace(dev, &ws))) return st;
-  // The scalar result lives at the end of the library workspace.
+  // The scalar result lives at the end of the library workspace.  Callers on
+  // other host threads share it and its partials, so the device lock is held
+  // from the launch until the result is on the host.
   void* res = static_cast<char*>(ws) + 1024 * sizeof(unsigned long long) + 64;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  std::lock_guard<std::mutex> lk(g_dev[dev].mu);
   if ((st = hipacc_hip_reduce(img, reduce, acc_dtype, res, ws, stream))) return st;
   hipError_t e = hipMemcpyAsync(result_host, res, 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -686,9 +797,10 @@ int hipacc_hip_histogram(const hipacc_view* img, int32_t num_bins, float scale, 
   a.vec_ok = rows_aligned(*img, 16);
   a.bins = bins_dev;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  void* ws = nullptr;
+  if ((st = stream_workspace(s, &ws))) return st;
+  a.partials = static_cast<uint32_t*>(ws);
   Timer tm(s);
-  st = launch_fill(bins_dev, 0u, (size_t)num_bins, 4, s);
-  if (st) return tm.finish(st);
   return tm.finish(launch_histogram(a, img->dtype, s));
 }
 
@@ -779,6 +891,9 @@ static int check_local_grad(const hipacc_local_grad_desc* d) {
   return HIPACC_OK;
 }
 
+static int run_local_grad(const hipacc_local_grad_desc* d, int variant, char* ws, bool timed,
+                          hipStream_t s);
+
 }  // namespace hh
 
 extern "C" {
@@ -798,16 +913,30 @@ int hipacc_hip_local_grad_variant(const hipacc_local_grad_desc* d, int32_t varia
   if (st) return st;
   if (variant != kVarAuto && variant != kVarTile && variant != kVarStream)
     return set_error(HIPACC_ERR_INVALID_ARG, "local_grad: variant %d", variant);
-  const bool dx = d->grad_in.data != nullptr, dw = d->grad_weights_dev != nullptr;
-  if (!dx && !dw) return HIPACC_OK;
-  const int kx = d->mask_w, ky = d->mask_h, kk = kx * ky;
-  const ConvGradWs lay = conv_grad_ws(d);
+  if (!d->grad_in.data && !d->grad_weights_dev) return HIPACC_OK;
   char* ws = static_cast<char*>(d->workspace);
   if (!ws) {
     void* p = nullptr;
-    if ((st = ensure_grad_workspace(lay.total, &p))) return st;
+    if ((st = ensure_grad_workspace(conv_grad_ws(d).total, &p))) return st;
     ws = static_cast<char*>(p);
   }
+  return run_local_grad(d, variant, ws, true, static_cast<hipStream_t>(stream));
+}
+
+
+}  // extern "C"
+
+namespace hh {
+
+// The adjoint's launches with workspace `ws` (conv_grad_ws(d).total bytes;
+// nothing in it needs to start at zero: every word is written before it is
+// read).  `timed`: honour hipacc_hip_set_timing.
+static int run_local_grad(const hipacc_local_grad_desc* d, int variant, char* ws, bool timed,
+                          hipStream_t s) {
+  int st = HIPACC_OK;
+  const bool dx = d->grad_in.data != nullptr, dw = d->grad_weights_dev != nullptr;
+  const int kx = d->mask_w, ky = d->mask_h, kk = kx * ky;
+  const ConvGradWs lay = conv_grad_ws(d);
   ConvGradArgs a;
   std::memset(&a, 0, sizeof(a));
   a.x = static_cast<const float*>(d->in.data);
@@ -831,8 +960,7 @@ int hipacc_hip_local_grad_variant(const hipacc_local_grad_desc* d, int32_t varia
   if (!a.wdev) std::memcpy(a.wh, d->weights_f32, sizeof(float) * kk);
   a.partials = reinterpret_cast<float*>(ws + lay.part);
   a.dw_variant = variant;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  Timer tm(s);
+  Timer tm(s, timed);
   if (dx) {
     // E over the image: the forward operator on dY with the flipped mask and
     // a zero border.
@@ -875,6 +1003,10 @@ int hipacc_hip_local_grad_variant(const hipacc_local_grad_desc* d, int32_t varia
   if (dw) st = launch_conv_dw(a, d->grad_weights_dev, s);
   return tm.finish(st);
 }
+
+}  // namespace hh
+
+extern "C" {
 
 // Spatial table and argument checks shared by the bilateral entry points.
 static int bilat_common(const hipacc_bilateral_grad_desc* d, float* sp) {

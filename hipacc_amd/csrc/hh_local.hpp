@@ -58,9 +58,20 @@ enum Variant {
 
 // Operator kinds the tuner keys on.
 enum TuneKind { kTuneLocal = 0, kTuneSeparable = 1, kTuneBox = 2 };
+// sz: size class of the iteration space, floor(log2(width * height *
+// batch)); a variant tuned at one size is not replayed at another.
 struct TuneKey {
-  int kind, kx, ky, in_dt, out_dt, acc_f, boundary, fma, wdev;
+  int kind, kx, ky, in_dt, out_dt, acc_f, boundary, fma, wdev, sz;
 };
+inline int tune_size_class(long long w, long long h, long long batch) {
+  unsigned long long n = (unsigned long long)(w * h * (batch > 1 ? batch : 1));
+  int c = 0;
+  while (n > 1) {
+    n >>= 1;
+    ++c;
+  }
+  return c;
+}
 struct TuneVal {
   int variant, param;
   float ms;
@@ -134,7 +145,10 @@ struct HistArgs {
   float scale;
   int vec_ok;
   uint32_t* bins;
+  uint32_t* partials;   // [blocks][nbins] per-block histograms (workspace)
 };
+// Entries of the histogram's partial-bin workspace: blocks * nbins <= this.
+constexpr int kHistPartialEntries = 1 << 18;
 int launch_histogram(const HistArgs& a, int dt, hipStream_t s);
 
 // ---- utilities --------------------------------------------------------------

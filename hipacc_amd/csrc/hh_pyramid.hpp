@@ -59,6 +59,62 @@ struct PyrImage {
 struct PyrTile {
   int img, x0, y0;
 };
+// The regions one tile of level l stages.  The tile covers L[l] pixels
+// [x0, x0+TW) x [y0, y0+TH) of a W x H level whose next level is W1 x H1.
+// u*: the G[l+1] points it needs -- the LF taps of its pixels plus the G[l+1]
+// block it owns (o*: columns x0/2 .. , rows y0/2 ..); r*: the G[l] region the
+// 5x5 blur taps of those points read (NN point of G[l+1] pixel u is
+// (int)interp_map(W / W1, u)).  The same float arithmetic runs on the host,
+// where the plan proves that every tile fits its kernel's LDS capacity.
+struct PyrRegion {
+  int ux0, uy0, uw, uh;
+  int ox0, ox1, oy0, oy1;
+  int rx0, ry0, rw, rh;
+};
+HH_HD PyrRegion pyr_tile_region(int W, int H, int W1, int H1, int x0, int y0, int TW, int TH) {
+  PyrRegion r;
+  const int xe = (x0 + TW < W ? x0 + TW : W) - 1, ye = (y0 + TH < H ? y0 + TH : H) - 1;
+  const float su = (float)W1 / (float)W, sv = (float)H1 / (float)H;
+  int ux0, ux1, uy0, uy1;
+  float f;
+  interp_lf(interp_map(su, x0), ux0, f);
+  interp_lf(interp_map(su, xe), ux1, f);
+  interp_lf(interp_map(sv, y0), uy0, f);
+  interp_lf(interp_map(sv, ye), uy1, f);
+  ux1 = ux1 + 1 < W1 - 1 ? ux1 + 1 : W1 - 1;
+  uy1 = uy1 + 1 < H1 - 1 ? uy1 + 1 : H1 - 1;
+  r.ox0 = x0 / 2;
+  r.ox1 = (x0 / 2 + TW / 2 < W1 ? x0 / 2 + TW / 2 : W1) - 1;
+  r.oy0 = y0 / 2;
+  r.oy1 = (y0 / 2 + TH / 2 < H1 ? y0 / 2 + TH / 2 : H1) - 1;
+  ux0 = ux0 < r.ox0 ? ux0 : r.ox0;
+  uy0 = uy0 < r.oy0 ? uy0 : r.oy0;
+  ux1 = ux1 > r.ox1 ? ux1 : r.ox1;
+  uy1 = uy1 > r.oy1 ? uy1 : r.oy1;
+  r.ux0 = ux0;
+  r.uy0 = uy0;
+  r.uw = ux1 - ux0 + 1;
+  r.uh = uy1 - uy0 + 1;
+  const float sd = (float)W / (float)W1, td = (float)H / (float)H1;
+  r.rx0 = (int)interp_map(sd, ux0) - 2;
+  r.ry0 = (int)interp_map(td, uy0) - 2;
+  r.rw = (int)interp_map(sd, ux1) + 2 - r.rx0 + 1;
+  r.rh = (int)interp_map(td, uy1) + 2 - r.ry0 + 1;
+  return r;
+}
+
+// LDS capacity of a TW x TH tile kernel whose images have NN step
+// s = W / W1 <= SNUM / 4: the G[l+1] region (UW x UH) and the G[l] region
+// its blur taps span, at most ceil(s*(UW-1)) + 5 by ceil(s*(UH-1)) + 5.
+template <int TW, int TH, int SNUM>
+struct PyrCap {
+  static constexpr int UW = TW / 2 + 4, UH = TH / 2 + 4;
+  static constexpr int RW = (SNUM * (UW - 1) + 3) / 4 + 5, RH = (SNUM * (UH - 1) + 3) / 4 + 5;
+  HH_HD static bool fits(const PyrRegion& r) {
+    return r.uw <= UW && r.uh <= UH && r.rw <= RW && r.rh <= RH;
+  }
+};
+
 struct PyrLevelArgs {
   const PyrImage* imgs;
   const PyrTile* tiles;

@@ -148,7 +148,7 @@ int launch_local(const LocalArgs& a, int acc_is_float, hipStream_t s, int varian
   f.use_weights = 1;
   if (variant != kVarAuto) return launch_local_variant(f, acc_is_float, s, variant, param);
   TuneKey k{kTuneLocal, a.mask_w, a.mask_h, a.g.in_dtype, a.g.out_dtype, acc_is_float,
-            a.g.boundary, a.fma, a.wdev != nullptr};
+            a.g.boundary, a.fma, a.wdev != nullptr, tune_size_class(a.g.is_w, a.g.is_h, a.g.batch)};
   TuneVal v;
   if (tune_lookup(k, v)) {
     if (v.variant == kVarGeneric) return launch_generic(a, acc_is_float, s);

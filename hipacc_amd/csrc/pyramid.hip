@@ -78,10 +78,8 @@ __device__ __forceinline__ float pyr_read(const float* g, int stride, int w, int
 // high at this level), which sizes the staged region.
 template <int TW, int TH, int SNUM, bool FMA>
 __global__ __launch_bounds__(256) void pyramid_level_kernel(const PyrLevelArgs a) {
-  constexpr int UW = TW / 2 + 4, UH = TH / 2 + 4;   // G[l+1] region capacity
-  // G[l] region capacity: the blur taps of UW x UH sample points, s apart,
-  // span at most ceil(s*(UW-1)) + 5 x ceil(s*(UH-1)) + 5.
-  constexpr int RW = (SNUM * (UW - 1) + 3) / 4 + 5, RH = (SNUM * (UH - 1) + 3) / 4 + 5;
+  using Cap = PyrCap<TW, TH, SNUM>;
+  constexpr int UW = Cap::UW, UH = Cap::UH, RW = Cap::RW, RH = Cap::RH;
   __shared__ float g1s[UH * UW];
   __shared__ float gls[RH * RW];
   const int l = a.level;
@@ -96,30 +94,16 @@ __global__ __launch_bounds__(256) void pyramid_level_kernel(const PyrLevelArgs a
     float* L = im.lap[l];
     const int gs = im.gs[l], g1st = im.gs[l + 1], ls = im.ls[l];
     const int x0 = tile.x0, y0 = tile.y0;
-    const int xe = min(x0 + TW, W) - 1, ye = min(y0 + TH, H) - 1;
-    // G[l+1] region: LF taps of the tile plus the owned block.
     const float su = (float)W1 / (float)W, sv = (float)H1 / (float)H;
-    int ux0, ux1, uy0, uy1;
-    float f;
-    interp_lf(interp_map(su, x0), ux0, f);
-    interp_lf(interp_map(su, xe), ux1, f);
-    interp_lf(interp_map(sv, y0), uy0, f);
-    interp_lf(interp_map(sv, ye), uy1, f);
-    ux1 = min(ux1 + 1, W1 - 1);
-    uy1 = min(uy1 + 1, H1 - 1);
-    const int ox0 = x0 / 2, ox1 = min(x0 / 2 + TW / 2, W1) - 1;
-    const int oy0 = y0 / 2, oy1 = min(y0 / 2 + TH / 2, H1) - 1;
-    ux0 = min(ux0, ox0);
-    uy0 = min(uy0, oy0);
-    ux1 = max(ux1, ox1);
-    uy1 = max(uy1, oy1);
-    const int uw = ux1 - ux0 + 1, uh = uy1 - uy0 + 1;
-    // G[l] region of the blur taps at the NN sample points
     const float sd = (float)W / (float)W1, td = (float)H / (float)H1;
-    const int rx0 = (int)interp_map(sd, ux0) - 2, rx1 = (int)interp_map(sd, ux1) + 2;
-    const int ry0 = (int)interp_map(td, uy0) - 2, ry1 = (int)interp_map(td, uy1) + 2;
-    const int rw = rx1 - rx0 + 1, rh = ry1 - ry0 + 1;
-    if (uw > UW || uh > UH || rw > RW || rh > RH) continue;   // cannot happen (see above)
+    const PyrRegion rg = pyr_tile_region(W, H, W1, H1, x0, y0, TW, TH);
+    const int ux0 = rg.ux0, uy0 = rg.uy0, uw = rg.uw, uh = rg.uh;
+    const int ox0 = rg.ox0, ox1 = rg.ox1, oy0 = rg.oy0, oy1 = rg.oy1;
+    const int rx0 = rg.rx0, ry0 = rg.ry0, rw = rg.rw, rh = rg.rh;
+    const int rx1 = rx0 + rw - 1, ry1 = ry0 + rh - 1;
+    // The plan verified on the host (pyr_plan_tiles) that every tile of its
+    // work lists fits; this guard only keeps LDS accesses in bounds.
+    if (!Cap::fits(rg)) continue;
 
     __syncthreads();  // previous tile's LDS reads are done
     // wave w stages rows w, w+4, ...; lane c the columns c, c+64, ...  Inside
@@ -256,6 +240,67 @@ static int plan_launches(hipacc_pyramid_plan* p, hipStream_t s) {
   return HIPACC_OK;
 }
 
+// Work lists of every level: images at least 9 x 9 at a level (NN step
+// <= 2.25) may take wide 128 x 32 tiles; a level takes them when there are
+// at least kPyrWideMin (they fill the device several times over), and the
+// smaller images of that level, or every image of a smaller level, take
+// narrow 64 x tile_rows tiles.  Every tile's staged regions are computed with
+// the kernel's own arithmetic and checked against the LDS capacity of the
+// kernel that will run it; a tile that does not fit is an error, never a
+// skipped tile.
+static int pyr_plan_tiles(const int* w0, const int* h0, int n, int levels, int tile_rows,
+                          std::vector<std::vector<PyrTile>>& lists,
+                          std::vector<std::vector<PyrTile>>& wlists) {
+  lists.assign(levels, {});
+  wlists.assign(levels, {});
+  std::vector<std::vector<PyrTile>> nlists(levels);
+  std::vector<int> W(n), H(n);
+  for (int i = 0; i < n; ++i) {
+    int w = w0[i], h = h0[i];
+    for (int l = 0; l < levels; ++l) {
+      if (w < 1 || h < 1)
+        return set_error(HIPACC_ERR_INVALID_ARG, "pyramid: image %d too small for %d levels", i,
+                         levels);
+      if (l + 1 < levels) {
+        if (w >= 9 && h >= 9)
+          for (int y0 = 0; y0 < h; y0 += 32)
+            for (int x0 = 0; x0 < w; x0 += 128) wlists[l].push_back(PyrTile{i, x0, y0});
+        for (int y0 = 0; y0 < h; y0 += tile_rows)
+          for (int x0 = 0; x0 < w; x0 += 64) nlists[l].push_back(PyrTile{i, x0, y0});
+      }
+      w /= 2;
+      h /= 2;
+    }
+  }
+  auto lw = [&](int i, int l) { return w0[i] >> l; };
+  auto lh = [&](int i, int l) { return h0[i] >> l; };
+  for (int l = 0; l + 1 < levels; ++l) {
+    if ((int)wlists[l].size() >= kPyrWideMin) {
+      for (const PyrTile& t : nlists[l])
+        if (lw(t.img, l) < 9 || lh(t.img, l) < 9) lists[l].push_back(t);
+    } else {
+      wlists[l].clear();
+      lists[l] = nlists[l];
+    }
+    for (int wide = 0; wide < 2; ++wide)
+      for (const PyrTile& t : wide ? wlists[l] : lists[l]) {
+        const int Wl = lw(t.img, l), Hl = lh(t.img, l);
+        const int TW = wide ? 128 : 64, TH = wide ? 32 : tile_rows;
+        const PyrRegion r = pyr_tile_region(Wl, Hl, Wl / 2, Hl / 2, t.x0, t.y0, TW, TH);
+        const bool ok = wide ? PyrCap<128, 32, 9>::fits(r)
+                        : tile_rows == 8  ? PyrCap<64, 8, 12>::fits(r)
+                        : tile_rows == 32 ? PyrCap<64, 32, 12>::fits(r)
+                                          : PyrCap<64, 16, 12>::fits(r);
+        if (!ok)
+          return set_error(HIPACC_ERR_INVALID_ARG,
+                           "pyramid: image %d level %d tile (%d, %d) stages %dx%d / %dx%d "
+                           "pixels, more than its kernel holds",
+                           t.img, l, t.x0, t.y0, r.uw, r.uh, r.rw, r.rh);
+      }
+  }
+  return HIPACC_OK;
+}
+
 static void plan_free(hipacc_pyramid_plan* p) {
   if (!p) return;
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
@@ -298,18 +343,24 @@ int hipacc_hip_pyramid_plan_create(const hipacc_pyramid_desc* d, void* stream,
   p->cnt.assign(d->levels, 0);
   p->offw.assign(d->levels, 0);
   p->cntw.assign(d->levels, 0);
-  std::vector<std::vector<PyrTile>> lists(d->levels), wlists(d->levels), nlists(d->levels);
+  std::vector<int> w0(d->num_images), h0(d->num_images);
+  for (int i = 0; i < d->num_images; ++i) {
+    w0[i] = d->images[i].width;
+    h0[i] = d->images[i].height;
+  }
+  std::vector<std::vector<PyrTile>> lists, wlists;
+  int st0 = pyr_plan_tiles(w0.data(), h0.data(), d->num_images, d->levels, p->tile_rows, lists,
+                           wlists);
+  if (st0) {
+    plan_free(p);
+    return st0;
+  }
   for (int i = 0; i < d->num_images; ++i) {
     const hipacc_pyramid_image& s = d->images[i];
     PyrImage& m = imgs[i];
     std::memset(&m, 0, sizeof(m));
     int w = s.width, h = s.height;
     for (int l = 0; l < d->levels; ++l) {
-      if (w < 1 || h < 1) {
-        plan_free(p);
-        return set_error(HIPACC_ERR_INVALID_ARG, "pyramid: image %d too small for %d levels", i,
-                         d->levels);
-      }
       if (!s.gauss[l] || s.gauss_stride[l] < w || (l + 1 < d->levels && (!s.lap[l] || s.lap_stride[l] < w))) {
         plan_free(p);
         return set_error(HIPACC_ERR_INVALID_ARG, "pyramid: image %d level %d buffers", i, l);
@@ -321,28 +372,9 @@ int hipacc_hip_pyramid_plan_create(const hipacc_pyramid_desc* d, void* stream,
       if (l + 1 < d->levels) {
         m.lap[l] = s.lap[l];
         m.ls[l] = s.lap_stride[l];
-        // images at least 9 x 9 at this level (NN step <= 2.25) may take
-        // wide tiles; the smallest take narrow ones
-        if (w >= 9 && h >= 9) {
-          for (int y0 = 0; y0 < h; y0 += 32)
-            for (int x0 = 0; x0 < w; x0 += 128) wlists[l].push_back(PyrTile{i, x0, y0});
-        }
-        for (int y0 = 0; y0 < h; y0 += p->tile_rows)
-          for (int x0 = 0; x0 < w; x0 += 64) nlists[l].push_back(PyrTile{i, x0, y0});
       }
       w /= 2;
       h /= 2;
-    }
-  }
-  // A level takes wide tiles when they fill the device several times over;
-  // a smaller level takes narrow tiles for every image (more blocks).
-  for (int l = 0; l < d->levels; ++l) {
-    if ((int)wlists[l].size() >= kPyrWideMin) {
-      for (const PyrTile& t : nlists[l])
-        if (imgs[t.img].w[l] < 9 || imgs[t.img].h[l] < 9) lists[l].push_back(t);
-    } else {
-      wlists[l].clear();
-      lists[l] = nlists[l];
     }
   }
   std::vector<PyrTile> all;
@@ -394,6 +426,26 @@ int hipacc_hip_pyramid_plan_create(const hipacc_pyramid_desc* d, void* stream,
     return st;
   }
   *out = p;
+  return HIPACC_OK;
+}
+
+int hipacc_hip_pyramid_plan_tiles(const int32_t* widths, const int32_t* heights,
+                                  int32_t num_images, int32_t levels, int32_t tile_rows,
+                                  int32_t* narrow_tiles, int32_t* wide_tiles) {
+  if (!widths || !heights || num_images < 1)
+    return set_error(HIPACC_ERR_INVALID_ARG, "pyramid_plan_tiles: no images");
+  if (levels < 2 || levels > HIPACC_MAX_LEVELS)
+    return set_error(HIPACC_ERR_INVALID_ARG, "pyramid_plan_tiles: levels %d", levels);
+  if (tile_rows == 0) tile_rows = 16;
+  if (tile_rows != 8 && tile_rows != 16 && tile_rows != 32)
+    return set_error(HIPACC_ERR_INVALID_ARG, "pyramid_plan_tiles: tile_rows %d", tile_rows);
+  std::vector<std::vector<PyrTile>> lists, wlists;
+  const int st = pyr_plan_tiles(widths, heights, num_images, levels, tile_rows, lists, wlists);
+  if (st) return st;
+  for (int l = 0; l + 1 < levels; ++l) {
+    if (narrow_tiles) narrow_tiles[l] = (int32_t)lists[l].size();
+    if (wide_tiles) wide_tiles[l] = (int32_t)wlists[l].size();
+  }
   return HIPACC_OK;
 }
 

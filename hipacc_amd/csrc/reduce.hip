@@ -206,8 +206,15 @@ int launch_reduce(const ReduceArgs& a, int dt, int acc_dt, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Histogram: LDS-privatised bins per block (ds_add_u32), merged into the
-// global bins with one atomic per non-empty bin per block.
+// Histogram: two launches and no atomics on global memory.
+//   hist_partial_kernel: each block counts its share of the window into an
+//   LDS histogram (ds_add_u32 per pixel), then stores it as one row of the
+//   partials workspace.
+//   hist_merge_kernel: one block per bin sums that bin over the blocks' rows
+//   (integer sums: exact in any order) and writes it to the output.
+// Merging with one global atomic per bin and block instead serialises the
+// blocks' adds on each bin's address (a device-scope atomic per block and
+// bin): with 512 blocks that merge dominated the launch.
 // ---------------------------------------------------------------------------
 template <typename T> __device__ __forceinline__ int bin_of(T p, int nbins, float scale);
 template <> __device__ __forceinline__ int bin_of<uint8_t>(uint8_t p, int nbins, float) {
@@ -222,92 +229,106 @@ template <> __device__ __forceinline__ int bin_of<float>(float p, int nbins, flo
   return (int)(uint32_t)f;
 }
 
+constexpr int kHistThreads = 512;
+
 template <typename T>
-__global__ __launch_bounds__(256) void histogram_kernel(const HistArgs a) {
+__global__ __launch_bounds__(kHistThreads) void hist_partial_kernel(const HistArgs a) {
   constexpr int VEC = 16 / (int)sizeof(T);
+  constexpr int NT = kHistThreads;
   extern __shared__ uint32_t hist[];
-  for (int i = threadIdx.x; i < a.nbins; i += 256) hist[i] = 0u;
+  for (int i = threadIdx.x; i < a.nbins; i += NT) hist[i] = 0u;
   __syncthreads();
+  auto count1 = [&](T p) {
+    const int b = bin_of<T>(p, a.nbins, a.scale);
+    if (b >= 0 && b < a.nbins) atomicAdd(&hist[b], 1u);
+  };
   auto count = [&](const uint4& w) {
     T t[VEC];
     __builtin_memcpy(t, &w, 16);
 #pragma unroll
-    for (int k = 0; k < VEC; ++k) {
-      const int b = bin_of<T>(t[k], a.nbins, a.scale);
-      if (b >= 0 && b < a.nbins) atomicAdd(&hist[b], 1u);
-    }
+    for (int k = 0; k < VEC; ++k) count1(t[k]);
   };
   if (a.vec_ok && a.stride == a.w) {
-    // rows back to back: one grid-stride sweep with 4 loads in flight per
-    // thread
+    // Rows back to back: block b takes the contiguous vectors
+    // [b*nv/G, (b+1)*nv/G), each thread U loads in flight.
     constexpr int U = 4;
     const T* base = static_cast<const T*>(a.in) + (long long)a.oy * a.stride + a.ox;
-    const long long nv = (long long)a.h * a.w / VEC;
-    const long long st = (long long)gridDim.x * 256;
+    const long long n = (long long)a.h * a.w;
+    const long long nv = n / VEC;
+    const long long b0 = nv * blockIdx.x / gridDim.x, b1 = nv * (blockIdx.x + 1) / gridDim.x;
     const uint4* v = reinterpret_cast<const uint4*>(base);
-    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    for (; i + (U - 1) * st < nv; i += U * st) {
+    long long i = b0 + threadIdx.x;
+    for (; i + (U - 1) * NT < b1; i += U * NT) {
       uint4 w[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) w[u] = v[i + u * st];
+      for (int u = 0; u < U; ++u) w[u] = v[i + u * NT];
 #pragma unroll
       for (int u = 0; u < U; ++u) count(w[u]);
     }
-    for (; i < nv; i += st) count(v[i]);
-    const long long n = (long long)a.h * a.w;
-    for (long long x = nv * VEC + (long long)blockIdx.x * 256 + threadIdx.x; x < n; x += st) {
-      const int b = bin_of<T>(base[x], a.nbins, a.scale);
-      if (b >= 0 && b < a.nbins) atomicAdd(&hist[b], 1u);
-    }
-  } else
-  for (int y = blockIdx.x; y < a.h; y += gridDim.x) {
-    const T* row = static_cast<const T*>(a.in) + (long long)(a.oy + y) * a.stride + a.ox;
-    int start = 0;
-    if (a.vec_ok) {
-      const int nv = a.w / VEC;
-      for (int i = threadIdx.x; i < nv; i += 256) {
-        const uint4 w = reinterpret_cast<const uint4*>(row)[i];
-        T t[VEC];
-        __builtin_memcpy(t, &w, 16);
-#pragma unroll
-        for (int k = 0; k < VEC; ++k) {
-          const int b = bin_of<T>(t[k], a.nbins, a.scale);
-          if (b >= 0 && b < a.nbins) atomicAdd(&hist[b], 1u);
-        }
+    for (; i < b1; i += NT) count(v[i]);
+    if (blockIdx.x == gridDim.x - 1)
+      for (long long x = nv * VEC + threadIdx.x; x < n; x += NT) count1(base[x]);
+  } else {
+    for (int y = blockIdx.x; y < a.h; y += gridDim.x) {
+      const T* row = static_cast<const T*>(a.in) + (long long)(a.oy + y) * a.stride + a.ox;
+      int start = 0;
+      if (a.vec_ok) {
+        const int nv = a.w / VEC;
+        for (int i = threadIdx.x; i < nv; i += NT) count(reinterpret_cast<const uint4*>(row)[i]);
+        start = nv * VEC;
       }
-      start = nv * VEC;
-    }
-    for (int x = start + threadIdx.x; x < a.w; x += 256) {
-      const int b = bin_of<T>(row[x], a.nbins, a.scale);
-      if (b >= 0 && b < a.nbins) atomicAdd(&hist[b], 1u);
+      for (int x = start + threadIdx.x; x < a.w; x += NT) count1(row[x]);
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < a.nbins; i += 256)
-    if (hist[i]) atomicAdd(&a.bins[i], hist[i]);
+  uint32_t* part = a.partials + (long long)blockIdx.x * a.nbins;
+  for (int i = threadIdx.x; i < a.nbins; i += NT) part[i] = hist[i];
+}
+
+__global__ __launch_bounds__(256) void hist_merge_kernel(const uint32_t* partials, int nparts,
+                                                         int nbins, uint32_t* bins) {
+  __shared__ uint32_t red[4];
+  const int b = blockIdx.x;
+  uint32_t s = 0;
+  for (int p = threadIdx.x; p < nparts; p += 256) s += partials[(long long)p * nbins + b];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) bins[b] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 int launch_histogram(const HistArgs& a, int dt, hipStream_t s) {
   const int eb = dt == HIPACC_U8 ? 1 : dt == HIPACC_U16 ? 2 : 4;
   const long long n = (long long)a.w * a.h;
-  // contiguous windows: 512 blocks (fewer per-block merges into the global
-  // bins); else one block per row, at most 1024
+  // Contiguous windows: about 32 KiB per block, at most 512 blocks (two per
+  // CU); other windows: one block per row.  blocks * nbins entries must fit
+  // the partials workspace.
   const bool flat = a.vec_ok && a.stride == a.w;
-  const long long nflat = (n * eb + 4095) / 4096;
-  const int blocks = flat ? (int)(nflat < 512 ? (nflat > 0 ? nflat : 1) : 512)
-                          : (a.h < kRedBlocks ? a.h : kRedBlocks);
+  const long long want = flat ? (n * eb + 32767) / 32768 : a.h;
+  long long blocks = want < 512 ? want : 512;
+  const long long cap = kHistPartialEntries / a.nbins;
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
   const size_t smem = (size_t)a.nbins * 4;
   switch (dt) {
     case HIPACC_U8:
-      hipLaunchKernelGGL(histogram_kernel<uint8_t>, dim3(blocks), dim3(256), smem, s, a); break;
+      hipLaunchKernelGGL(hist_partial_kernel<uint8_t>, dim3(blocks), dim3(kHistThreads), smem, s, a);
+      break;
     case HIPACC_U16:
-      hipLaunchKernelGGL(histogram_kernel<uint16_t>, dim3(blocks), dim3(256), smem, s, a); break;
+      hipLaunchKernelGGL(hist_partial_kernel<uint16_t>, dim3(blocks), dim3(kHistThreads), smem, s, a);
+      break;
     case HIPACC_F32:
-      hipLaunchKernelGGL(histogram_kernel<float>, dim3(blocks), dim3(256), smem, s, a); break;
+      hipLaunchKernelGGL(hist_partial_kernel<float>, dim3(blocks), dim3(kHistThreads), smem, s, a);
+      break;
     default:
       return set_error(HIPACC_ERR_UNSUPPORTED, "histogram: pixel type must be U8, U16 or F32");
   }
-  return hip_status(hipGetLastError(), "histogram_kernel launch");
+  int st = hip_status(hipGetLastError(), "hist_partial_kernel launch");
+  if (st) return st;
+  hipLaunchKernelGGL(hist_merge_kernel, dim3(a.nbins), dim3(256), 0, s, a.partials, (int)blocks,
+                     a.nbins, a.bins);
+  return hip_status(hipGetLastError(), "hist_merge_kernel launch");
 }
 
 // ---------------------------------------------------------------------------

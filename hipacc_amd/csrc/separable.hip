@@ -226,7 +226,8 @@ int launch_separable(const SepArgs& a, int acc_is_float, int fp_contract, hipStr
                      int variant, int param) {
   if (variant != kVarAuto) return launch_sep_variant(a, acc_is_float, fp_contract, s, variant, param);
   TuneKey k{a.box ? kTuneBox : kTuneSeparable, a.kx, a.ky, a.g.in_dtype, a.g.out_dtype,
-            acc_is_float, a.g.boundary, fp_contract, 0};
+            acc_is_float, a.g.boundary, fp_contract, 0,
+            tune_size_class(a.g.is_w, a.g.is_h, a.g.batch)};
   TuneVal v;
   if (tune_lookup(k, v)) {
     const int st = launch_sep_variant(a, acc_is_float, fp_contract, s, v.variant, v.param);

@@ -352,6 +352,21 @@ def pyramid_shapes(h: int, w: int, levels: int):
     return [(h >> l, w >> l) for l in range(levels)]
 
 
+def pyramid_plan_tiles(shapes, levels: int, tile_rows: int = 0):
+    """Narrow and wide tile counts per level of the work lists a pyramid plan
+    builds for level-0 shapes [(h, w), ...]; host only, no device call.
+    Raises when a tile would not fit its kernel's LDS capacity."""
+    n = len(shapes)
+    hs = np.array([h for h, _ in shapes], np.int32)
+    ws = np.array([w for _, w in shapes], np.int32)
+    narrow = np.zeros(max(levels - 1, 1), np.int32)
+    wide = np.zeros(max(levels - 1, 1), np.int32)
+    capi.check(capi.lib().hipacc_hip_pyramid_plan_tiles(
+        ws.ctypes.data, hs.ctypes.data, n, levels, tile_rows, narrow.ctypes.data,
+        wide.ctypes.data), "hipacc_hip_pyramid_plan_tiles")
+    return narrow[:levels - 1].tolist(), wide[:levels - 1].tolist()
+
+
 class PyramidPlan:
     """Batched Gaussian / Laplacian pyramids of many images.
 

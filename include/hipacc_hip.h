@@ -21,10 +21,13 @@
  *  - `stream` is a hipStream_t passed as void* (NULL = the null stream).
  *    Launches are asynchronous on that stream; nothing here synchronises
  *    unless its comment says so.
- *  - Calls on different streams may run concurrently; calls that use the
- *    library's internal workspace (reductions, histograms, weight gradients)
- *    serialise that workspace per device with a mutex at enqueue time only,
- *    so concurrent calls on different streams must pass their own workspace.
+ *  - Calls on different streams may run concurrently.  Histograms use a
+ *    workspace per (device, stream), so they are reentrant per stream.
+ *    hipacc_hip_reduce_sync holds its device's lock until its result is on
+ *    the host.  The other users of a per-device library workspace
+ *    (hipacc_hip_reduce and the adjoints with a NULL workspace) take the
+ *    lock at enqueue time only: concurrent calls of those on different
+ *    streams must pass their own workspace.
  */
 #ifndef HIPACC_HIP_H
 #define HIPACC_HIP_H
@@ -232,7 +235,10 @@ int hipacc_hip_reduce_sync(const hipacc_view* img, int32_t reduce,
  * for U16; idx = (uint)(p * scale) for F32 (p < 0 or NaN dropped).  Indices
  * >= num_bins are dropped, as the generated CPU code does
  * (runtime/hipacc_cpu_red.hpp:71-76).  bins_dev (device, num_bins uint32)
- * is overwritten.  Replaces hipaccApplyBinningSegmented
+ * is overwritten (every bin is written; it need not be cleared).  Two
+ * launches: per-block LDS histograms into a workspace of the (device,
+ * stream) pair, then one block per bin sums them.
+ * Replaces hipaccApplyBinningSegmented
  * (runtime/hipacc_cu.tpp:410-464) and hipacc_binning_reduction
  * (runtime/hipacc_cu_red.hpp:495-641).                                      */
 #define HIPACC_MAX_BINS 4096
@@ -299,6 +305,7 @@ int hipacc_hip_local_grad(const hipacc_local_grad_desc* desc, void* stream);
  * the same dX; dw agrees to rounding (each sums in its own fixed order).    */
 int hipacc_hip_local_grad_variant(const hipacc_local_grad_desc* desc, int32_t variant,
                                   void* stream);
+
 
 /* ---- bilateral filter with a learnable range sigma -------------------- */
 /* Training forward: hipacc_hip_bilateral (F32, whole images) that also
@@ -410,6 +417,16 @@ int hipacc_hip_pyramid_run(hipacc_pyramid_plan* plan, void* stream);
 /* The same launches issued directly, without the graph.                    */
 int hipacc_hip_pyramid_run_direct(hipacc_pyramid_plan* plan, void* stream);
 int hipacc_hip_pyramid_plan_destroy(hipacc_pyramid_plan* plan);
+/* Host only (no device call): builds the work lists that
+ * hipacc_hip_pyramid_plan_create builds for these level-0 sizes, proves that
+ * every tile's staged regions fit the LDS capacity of the kernel that runs
+ * it (HIPACC_ERR_INVALID_ARG otherwise, exactly as plan_create fails), and
+ * writes the narrow and wide tile count of each level (levels-1 entries;
+ * either pointer may be NULL).  An inspection helper of the plan; the
+ * reference's pyramids (runtime/hipacc_base.hpp:131-191) have no work list. */
+int hipacc_hip_pyramid_plan_tiles(const int32_t* widths, const int32_t* heights,
+                                  int32_t num_images, int32_t levels, int32_t tile_rows,
+                                  int32_t* narrow_tiles, int32_t* wide_tiles);
 
 /* ---- kernel variants and the autotuner ------------------------------- */
 /* A local or separable operator runs one of these kernel variants:
@@ -426,10 +443,15 @@ int hipacc_hip_pyramid_plan_destroy(hipacc_pyramid_plan* plan);
  * All variants compute identical results.  The plain entry points use the
  * tuned variant of the operator's shape when the tune cache has one, else
  * ring (for 8-bit local operators only from 2977 columns up), else stream,
- * else tile, else generic.  The tuner times every candidate on the
- * caller's buffers (the output is overwritten) and records the fastest per
- * (kind, mask size, pixel types, accumulator, boundary, contraction, device
- * weights).  The cache lives in the process and can be saved / loaded.
+ * else tile, else generic.  The tuner runs every candidate once on the
+ * caller's buffers (the output is overwritten), then times it back to back
+ * over buffer sets of the operator's extents that together exceed the
+ * Infinity Cache (512 MiB, allocated for the call; the caller's buffers
+ * alone when one set is below 8 MiB), and records the fastest per (kind,
+ * mask size, pixel types, accumulator, boundary, contraction, device
+ * weights, size class = floor(log2(width * height * batch)) of the
+ * iteration space).  The cache lives in the process and can be saved /
+ * loaded.
  * Replaces the reference's fixed per-device kernel configuration
  * (lib/DSL/ClassRepresentation.cpp:256-489 setDefaultConfig, -use-config /
  * -pixels-per-thread, include/hipacc/Device/TargetDescription.h:64-180).   */
@@ -437,6 +459,7 @@ typedef struct hipacc_tune_entry {
   int32_t kind;        /* 0 local weighted SUM, 1 separable, 2 box sum         */
   int32_t kx, ky;
   int32_t in_dtype, out_dtype, acc_dtype, boundary, fp_contract, weights_dev;
+  int32_t size_class;  /* floor(log2(width * height * batch))                 */
   int32_t variant;     /* 1 tile, 2 stream, 3 generic, 4 wide stream, 5 ring  */
   int32_t param;
   float   ms;          /* measured time per launch                            */
@@ -490,7 +513,10 @@ int hipacc_hip_fill_random(const hipacc_view* img, uint64_t seed, void* stream);
 
 /* hipacc_last_kernel_timing (runtime/hipacc_base.hpp:54-66): when timing is
  * enabled, each entry point brackets its launches with hipEvents on the
- * stream and synchronises, and the elapsed milliseconds are kept here.     */
+ * stream and synchronises, and the elapsed milliseconds are kept here.  The
+ * switch and the last time are per host thread, as the reference's timing
+ * singleton is thread_local (runtime/hipacc_base_standalone.hpp:35-39).  A
+ * call that fails leaves the last time unchanged.                          */
 int   hipacc_hip_set_timing(int enabled);
 float hipacc_hip_last_kernel_timing(void);
 

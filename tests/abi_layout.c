@@ -23,6 +23,9 @@ int main(void) {
   F(hipacc_bilateral_grad_desc, ksize); F(hipacc_bilateral_grad_desc, spatial);
   F(hipacc_bilateral_grad_desc, const_val); F(hipacc_bilateral_grad_desc, grad_sigma_r_dev);
   F(hipacc_bilateral_grad_desc, workspace);
+  F(hipacc_tune_entry, weights_dev); F(hipacc_tune_entry, size_class);
+  F(hipacc_tune_entry, variant); F(hipacc_tune_entry, ms);
+  printf("\"sizeof.tune\": %zu,\n", sizeof(hipacc_tune_entry));
   printf("\"sizeof.lgrad\": %zu, \"sizeof.bgrad\": %zu,\n", sizeof(hipacc_local_grad_desc),
          sizeof(hipacc_bilateral_grad_desc));
   printf("\"sizeof.view\": %zu, \"sizeof.local\": %zu, \"sizeof.sep\": %zu, \"sizeof.point\": %zu, \"sizeof.bilat\": %zu\n}\n",

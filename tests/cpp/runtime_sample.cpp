@@ -56,6 +56,15 @@ int main(int argc, char** argv) {
   int st = hipaccLaunchLocal(hipaccMakeAccessor(in8), hipaccMakeAccessor(gauss), gmask, gopt);
   save("gauss_u8", hipaccReadMemory(gauss), (size_t)W * H);
 
+  // Kernel timing (hipacc_last_kernel_timing): one timed launch, then one
+  // with timing off, which leaves the last time as it was.
+  hipaccSetTiming(true);
+  st |= hipaccLaunchLocal(hipaccMakeAccessor(in8), hipaccMakeAccessor(gauss), gmask, gopt);
+  const float t_on = hipacc_last_kernel_timing();
+  hipaccSetTiming(false);
+  st |= hipaccLaunchLocal(hipaccMakeAccessor(in8), hipaccMakeAccessor(gauss), gmask, gopt);
+  const float t_off = hipacc_last_kernel_timing();
+
   // Sobel-x with a Domain from the mask (holes at the zero column), int
   // accumulation, int16 output, MIRROR.
   const int sx[9] = {-1, 0, 1, -2, 0, 2, -1, 0, 1};
@@ -134,9 +143,9 @@ int main(int argc, char** argv) {
   std::FILE* f = std::fopen((g_out + "/scalars.txt").c_str(), "w");
   if (!f) return 1;
   std::fprintf(f, "status %d\nfsum %.9g\nfmin %.9g\nfmax %.9g\nusum %u\nvisited %d\n"
-               "pyr3 %d %d\nstride_gauss %d\nstride_sobel %d\n",
+               "pyr3 %d %d\nstride_gauss %d\nstride_sobel %d\ntiming %.9g %.9g\n",
                st, fsum, fmin, fmax, usum, visited, last->get_width(), last->get_height(),
-               gauss->get_stride(), sob->get_stride());
+               gauss->get_stride(), sob->get_stride(), t_on, t_off);
   std::fclose(f);
   return st == 0 ? 0 : 1;
 }

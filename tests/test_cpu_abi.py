@@ -39,14 +39,16 @@ def test_struct_layout_matches_ctypes(tmp_path):
           "hipacc_separable_desc": capi.SeparableDesc, "hipacc_point_desc": capi.PointDesc,
           "hipacc_bilateral_desc": capi.BilateralDesc,
           "hipacc_local_grad_desc": capi.LocalGradDesc,
-          "hipacc_bilateral_grad_desc": capi.BilateralGradDesc}
+          "hipacc_bilateral_grad_desc": capi.BilateralGradDesc,
+          "hipacc_tune_entry": capi.TuneEntry}
     rename = {"in": "input", "out": "output"}
     for key, off in lay.items():
         t, m = key.split(".")
         if t == "sizeof":
             cls = {"view": capi.View, "local": capi.LocalDesc, "sep": capi.SeparableDesc,
                    "point": capi.PointDesc, "bilat": capi.BilateralDesc,
-                   "lgrad": capi.LocalGradDesc, "bgrad": capi.BilateralGradDesc}[m]
+                   "lgrad": capi.LocalGradDesc, "bgrad": capi.BilateralGradDesc,
+                   "tune": capi.TuneEntry}[m]
             assert ctypes.sizeof(cls) == off, key
         else:
             assert getattr(py[t], rename.get(m, m)).offset == off, key

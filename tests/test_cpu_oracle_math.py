@@ -7,6 +7,8 @@ reference DSL fixtures:
 * correlation orientation (mask index [dy+r][dx+r] multiplies in(x+dx, y+dy)),
 * the two float rounding orders (mul-then-add, and fmaf) by exact rational
   arithmetic on sampled pixels,
+* the float separable program (row pass into a float32 intermediate, then
+  column pass) by exact rational arithmetic and against FP64 correlate1d,
 * integer kernels, separable == 2D outer product, reductions, histograms,
   bilateral and point operators by brute force.
 A planted mistake (wrong reflection, transposed mask, dropped tap, FMA in the
@@ -22,6 +24,7 @@ import scipy.ndimage as ndi
 from hipacc_amd import capi, masks
 
 import oracle_binding as ob
+from weights import skewed_1d
 
 SCIPY_MODE = {capi.CLAMP: "nearest", capi.REPEAT: "wrap", capi.MIRROR: "reflect",
               capi.CONSTANT: "constant"}
@@ -186,3 +189,101 @@ def test_output_conversion_semantics():
     np.testing.assert_array_equal(out[0], [0, 1, 255, 255, 0, 0, 45, 0])
     ob.point(capi.POINT_COPY, vals, out, cvt=capi.CVT_TRUNC)
     np.testing.assert_array_equal(out[0], [0, 0, 254, 255, 255, 0, 44, 0])
+
+
+# ---- float separable: both passes pinned independently of the oracle ------
+SEP_MODES = [capi.CLAMP, capi.REPEAT, capi.MIRROR, capi.CONSTANT]
+
+
+def _index_map(n: int, r: int, mode: int) -> np.ndarray:
+    """Source index of each padded position -r .. n+r-1 (numpy.pad semantics
+    for the mode; -1 marks the CONSTANT value)."""
+    if mode == capi.CONSTANT:
+        return np.pad(np.arange(n), r, mode="constant", constant_values=-1)
+    return np.pad(np.arange(n), r, mode=PAD_MODE[mode])
+
+
+def _exact_sum(weights, values, contract):
+    """Row-major accumulation of w*v as float32 code does it: the first
+    product seeds the sum, then either one rounding per fused multiply-add
+    (contract) or a rounded product followed by a rounded add."""
+    acc = None
+    for wv, xv in zip(weights, values):
+        pw, px = Fraction(float(wv)), Fraction(float(xv))
+        if acc is None:
+            acc = _round_f32(pw * px)
+        elif contract:
+            acc = _round_f32(pw * px + Fraction(float(acc)))
+        else:
+            p = _round_f32(pw * px)
+            acc = _round_f32(Fraction(float(p)) + Fraction(float(acc)))
+    return acc
+
+
+@pytest.mark.parametrize("mode", SEP_MODES)
+@pytest.mark.parametrize("kx,ky", [(13, 7), (5, 9), (3, 3)])
+@pytest.mark.parametrize("contract", [False, True])
+def test_separable_float_rounding_order_exact(mode, kx, ky, contract):
+    """Each output is the column pass over float32 intermediate pixels, each
+    intermediate pixel the row pass over input pixels, both accumulated in
+    the stated rounding order; intermediate rows outside the window read the
+    CONSTANT value.  Row and column taps differ and kx != ky, so swapping
+    the passes or the tap order fails."""
+    H, W = 11, 17
+    src = ob.random_image((H, W), np.float32, 100 + kx)
+    row = skewed_1d(kx, 7 + kx, signed=True)
+    col = skewed_1d(ky, 9 + ky, signed=True)
+    cval = 0.375
+    dst = np.zeros_like(src)
+    ob.separable(src, dst, row, col, boundary=mode, const_val=cval, fp_contract=contract)
+    rx, ry = kx // 2, ky // 2
+    xm, ym = _index_map(W, rx, mode), _index_map(H, ry, mode)
+
+    def inter(yy, x):  # intermediate pixel (row pass) at source row yy
+        if yy < 0:
+            return np.float32(cval)
+        vals = [np.float32(cval) if xm[x + dx] < 0 else src[yy, xm[x + dx]] for dx in range(kx)]
+        return _exact_sum(row, vals, contract)
+
+    for (y, x) in ((0, 0), (0, W - 1), (H - 1, 0), (H - 1, W - 1), (1, 8), (5, 2), (6, 9),
+                   (H - 2, W - 3), (3, W - 1)):
+        hv = [inter(ym[y + dy], x) for dy in range(ky)]
+        assert dst[y, x] == _exact_sum(col, hv, contract), (y, x)
+
+
+@pytest.mark.parametrize("mode", SEP_MODES)
+@pytest.mark.parametrize("kx,ky", [(13, 7), (7, 13), (9, 9)])
+@pytest.mark.parametrize("contract", [False, True])
+def test_separable_float_vs_fp64_correlate1d(mode, kx, ky, contract):
+    """The whole image against scipy's FP64 correlate1d along x, then along
+    y (the same mode and constant for the intermediate), within the bound of
+    float32 sequential summation: |err| <= (g_kx + g_ky + g_kx*g_ky) *
+    sum|row| * sum|col| * max|x|, g_n = n u / (1 - n u), u = 2^-24.  A
+    swapped or transposed pass misses by orders of magnitude more."""
+    H, W = 40, 57
+    src = ob.random_image((H, W), np.float32, 200 + kx + ky)
+    row = skewed_1d(kx, 31 + kx, signed=True)
+    col = skewed_1d(ky, 37 + ky, signed=True)
+    cval = -0.625
+    dst = np.zeros_like(src)
+    ob.separable(src, dst, row, col, boundary=mode, const_val=cval, fp_contract=contract)
+    m = SCIPY_MODE[mode]
+    ref = ndi.correlate1d(src.astype(np.float64), row.astype(np.float64), axis=1, mode=m,
+                          cval=cval)
+    ref = ndi.correlate1d(ref, col.astype(np.float64), axis=0, mode=m, cval=cval)
+    u = 2.0 ** -24
+
+    def g(n):
+        return n * u / (1 - n * u)
+
+    mx = max(float(np.abs(src).max()), abs(cval))
+    bound = (g(kx) + g(ky) + g(kx) * g(ky)) * float(np.abs(row).sum()) * \
+        float(np.abs(col).sum()) * mx
+    err = float(np.abs(dst.astype(np.float64) - ref).max())
+    assert err <= bound, (err, bound)
+    # the transposed program is far outside the bound
+    swapped = np.zeros_like(src)
+    if kx != ky:
+        return
+    ob.separable(src, swapped, col, row, boundary=mode, const_val=cval, fp_contract=contract)
+    assert float(np.abs(swapped.astype(np.float64) - ref).max()) > 100 * bound

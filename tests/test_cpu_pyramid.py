@@ -52,3 +52,45 @@ def test_nn_downsample_of_even_size_picks_odd_pixels():
     dst = np.zeros((6, 8), np.float32)
     ob.resample(src, dst, capi.INTERP_NN)
     np.testing.assert_array_equal(dst, src[1::2, 1::2])
+
+
+# ---- the plan's work lists (host only) -------------------------------------
+def _bench_shapes():
+    import bench
+    return bench.pyramid_workload_shapes()
+
+
+@pytest.mark.parametrize("tile_rows", [8, 16, 32])
+def test_plan_tiles_fit_on_benchmark_geometry(tile_rows):
+    """Every tile of the 64-image, 8-level benchmark pyramid fits the LDS
+    capacity of the kernel that runs it (the library proves it on the host
+    and fails the plan otherwise), and levels 0-3 run on wide tiles."""
+    from hipacc_amd import operators as op
+    narrow, wide = op.pyramid_plan_tiles(_bench_shapes(), 8, tile_rows)
+    assert all(w >= 512 for w in wide[:4]), wide
+    assert all(w == 0 for w in wide[4:]), wide
+    shapes = _bench_shapes()
+    for l in range(7):
+        # every pixel of level l is covered exactly once by the two lists
+        if wide[l]:
+            assert wide[l] == sum(-(-(h >> l) // 32) * -(-(w >> l) // 128) for h, w in shapes
+                                  if (h >> l) >= 9 and (w >> l) >= 9)
+        else:
+            assert narrow[l] == sum(-(-(h >> l) // tile_rows) * -(-(w >> l) // 64)
+                                    for h, w in shapes)
+
+
+def test_plan_tiles_mixed_threshold_geometry():
+    """The GPU test geometry: two 2048^2 images fill level 1 with 512 wide
+    tiles, next to images of 8 and 9 rows at that level."""
+    from hipacc_amd import operators as op
+    import test_gpu_pyramid_wide as tw
+    narrow, wide = op.pyramid_plan_tiles(tw.MIXED, tw.MIXED_LEVELS)
+    assert wide[0] >= 512 and wide[1] >= 512, wide
+    assert narrow[1] > 0, narrow   # the 8-row image of level 1 takes narrow tiles
+
+
+def test_plan_tiles_rejects_bad_sizes():
+    from hipacc_amd import operators as op
+    with pytest.raises(capi.HipaccError):
+        op.pyramid_plan_tiles([(3, 300)], 4)   # no pixel left at level 2

@@ -9,6 +9,7 @@ from hipacc_amd import capi, masks, operators as op
 
 import oracle_binding as ob
 from gpu_util import empty_gpu, to_gpu, to_host
+from weights import skewed_1d
 
 pytestmark = pytest.mark.gpu
 
@@ -49,15 +50,19 @@ def test_sobel_f32_full(gpu, f32_image):
 
 
 @pytest.mark.parametrize("dt", [np.float32, np.uint8])
-def test_gauss13_separable_full(gpu, u8_image, f32_image, dt):
-    print(f"full-size 13-tap separable {np.dtype(dt)}", flush=True)
+@pytest.mark.parametrize("skew", [False, True])
+def test_gauss13_separable_full(gpu, u8_image, f32_image, dt, skew):
+    """The benchmarked 13-tap Gaussian, and (skew) the same row with a
+    different column, which pins the orientation of the two passes."""
+    print(f"full-size 13-tap separable {np.dtype(dt)} skew={skew}", flush=True)
     src = f32_image if dt == np.float32 else u8_image
     g13 = masks.gaussian_1d(13, 3.0)
+    c13 = skewed_1d(13, 1313) if skew else g13
     cvt = capi.CVT_ROUND if dt == np.uint8 else capi.CVT_TRUNC
     ref = np.zeros((N, N), dt)
-    ob.separable(src, ref, g13, g13, boundary=capi.CLAMP, cvt=cvt)
+    ob.separable(src, ref, g13, c13, boundary=capi.CLAMP, cvt=cvt)
     dst = empty_gpu((N, N), dt)
-    op.prepare_separable(to_gpu(src), dst, g13, g13, boundary=capi.CLAMP, cvt=cvt).launch()
+    op.prepare_separable(to_gpu(src), dst, g13, c13, boundary=capi.CLAMP, cvt=cvt).launch()
     np.testing.assert_array_equal(to_host(dst, dt), ref)
 
 

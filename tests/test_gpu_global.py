@@ -11,6 +11,7 @@ from hipacc_amd import capi, masks, operators as op
 import golden_cases as gc
 import oracle_binding as ob
 from gpu_util import empty_gpu, to_gpu, to_host
+from weights import skewed_1d
 
 pytestmark = pytest.mark.gpu
 
@@ -19,20 +20,24 @@ MODES = [capi.CLAMP, capi.REPEAT, capi.MIRROR, capi.CONSTANT]
 
 # ---- separable -----------------------------------------------------------
 @pytest.mark.parametrize("mode", MODES + [capi.UNDEFINED])
-@pytest.mark.parametrize("k", [3, 5, 7, 9, 13])
-def test_separable_f32_vs_two_pass_oracle(gpu, mode, k):
-    print(f"separable f32 k={k} mode={mode}", flush=True)
-    src = ob.random_image((77, 530), np.float32, 40 + k)
-    g = masks.gaussian_1d(k, k / 4.0)
+@pytest.mark.parametrize("kx,ky", [(3, 3), (5, 5), (7, 7), (9, 9), (13, 13), (13, 7), (3, 9)])
+def test_separable_f32_vs_two_pass_oracle(gpu, mode, kx, ky):
+    """Row and column taps differ (a Gaussian row, a signed random column),
+    so a swapped or transposed pass fails; kx != ky takes the shapes the
+    square-mask kernels refuse."""
+    print(f"separable f32 kx={kx} ky={ky} mode={mode}", flush=True)
+    src = ob.random_image((77, 530), np.float32, 40 + kx + ky)
+    g = masks.gaussian_1d(kx, kx / 4.0)
+    c = skewed_1d(ky, 50 + ky, signed=True)
     for contract in (False, True):
         ref = np.zeros_like(src)
-        ob.separable(src, ref, g, g, boundary=mode, const_val=0.3, fp_contract=contract)
+        ob.separable(src, ref, g, c, boundary=mode, const_val=0.3, fp_contract=contract)
         dst = empty_gpu(src.shape, np.float32)
-        op.separable(to_gpu(src), dst, g, g, boundary=mode, const_val=0.3, fp_contract=contract)
+        op.separable(to_gpu(src), dst, g, c, boundary=mode, const_val=0.3, fp_contract=contract)
         got = to_host(dst, np.float32)
         if mode == capi.UNDEFINED:
-            r = k // 2
-            got, ref = got[r:-r, r:-r], ref[r:-r, r:-r]
+            rx, ry = kx // 2, ky // 2
+            got, ref = got[ry:-ry, rx:-rx], ref[ry:-ry, rx:-rx]
         np.testing.assert_array_equal(got, ref)
 
 
@@ -40,10 +45,11 @@ def test_separable_f32_vs_two_pass_oracle(gpu, mode, k):
 def test_separable_u8_gaussian13(gpu, mode):
     src = ob.random_image((64, 2100), np.uint8, 77)
     g = masks.gaussian_1d(13, 3.0)
+    c = skewed_1d(13, 78)  # a different column: pins the pass orientation
     ref = np.zeros_like(src)
-    ob.separable(src, ref, g, g, boundary=mode, cvt=capi.CVT_ROUND, const_val=9)
+    ob.separable(src, ref, g, c, boundary=mode, cvt=capi.CVT_ROUND, const_val=9)
     dst = empty_gpu(src.shape, np.uint8)
-    op.separable(to_gpu(src), dst, g, g, boundary=mode, cvt=capi.CVT_ROUND, const_val=9)
+    op.separable(to_gpu(src), dst, g, c, boundary=mode, cvt=capi.CVT_ROUND, const_val=9)
     np.testing.assert_array_equal(to_host(dst, np.uint8), ref)
 
 

@@ -115,3 +115,13 @@ def test_runtime_pyramid_traversal(results):
     w3, h3 = W >> 3, H >> 3
     assert scal["pyr3"] == [str(w3), str(h3)]
     assert np.array_equal(_load(out, "pyr3_f32", np.float32, (h3, w3)), f32[:h3, :w3])
+
+
+@pytest.mark.gpu
+def test_runtime_kernel_timing(results):
+    """hipaccSetTiming(true): the launch is timed (a positive time);
+    hipaccSetTiming(false): hipacc_last_kernel_timing keeps the last time."""
+    out, scal = results
+    t_on, t_off = (float(v) for v in scal["timing"])
+    assert t_on > 0.0
+    assert t_off == t_on

@@ -10,6 +10,7 @@ from hipacc_amd import capi, masks, operators as op
 
 import oracle_binding as ob
 from gpu_util import empty_gpu, to_gpu, to_host
+from weights import skewed_1d
 
 pytestmark = pytest.mark.gpu
 
@@ -101,11 +102,12 @@ def test_int_sobel_variants(gpu, mode, out):
 def test_separable_variants(gpu, mode, k, dt):
     src = ob.random_image((70, 1100), dt, 31 + k)
     g1 = masks.gaussian_1d(k, k / 4.0)
+    c1 = skewed_1d(k, 60 + k)  # column taps differ from the row taps
     cvt = capi.CVT_ROUND if dt == np.uint8 else capi.CVT_TRUNC
     ref = np.zeros_like(src)
-    ob.separable(src, ref, g1, g1, boundary=mode, const_val=3, cvt=cvt)
+    ob.separable(src, ref, g1, c1, boundary=mode, const_val=3, cvt=cvt)
     dst = empty_gpu(src.shape, dt)
-    call = op.prepare_separable(to_gpu(src), dst, g1, g1, boundary=mode, const_val=3, cvt=cvt)
+    call = op.prepare_separable(to_gpu(src), dst, g1, c1, boundary=mode, const_val=3, cvt=cvt)
     r = k // 2
 
     def check(v, p):
@@ -160,10 +162,25 @@ def test_tuner_records_a_winner_and_auto_uses_it(gpu, tmp_path):
     call = op.prepare_local(to_gpu(src), dst, 5, 5, weights=w, boundary=capi.CLAMP)
     e = call.tune(iters=3)
     assert e.variant in capi.VARIANT_NAMES and e.ms > 0
+    assert e.size_class == 19   # 512 * 1024 = 2^19 pixels
+    # a large image (64 MiB per buffer set: the tuner rotates over 8 sets,
+    # 512 MiB) gets its own entry in its own size class
+    big = ob.random_image((2048, 4096), np.float32, 10)
+    dbig = empty_gpu(big.shape, np.float32)
+    cbig = op.prepare_local(to_gpu(big), dbig, 5, 5, weights=w, boundary=capi.CLAMP)
+    eb = cbig.tune(iters=8)
+    assert eb.size_class == 23 and eb.variant in capi.VARIANT_NAMES and eb.ms > 0
     path = str(tmp_path / "tune.txt").encode()
     capi.check(capi.lib().hipacc_hip_tune_save(path), "tune_save")
+    lines = [ln.split() for ln in open(path.decode()) if not ln.startswith("#")]
+    classes = {int(ln[9]) for ln in lines if ln[0] == "0" and ln[1] == "5" and ln[6] == "1"}
+    assert {19, 23} <= classes, lines
     capi.check(capi.lib().hipacc_hip_tune_load(path), "tune_load")
     call.launch()
+    cbig.launch()
     ref = np.zeros_like(src)
     ob.local(src, ref, 5, 5, weights=w, boundary=capi.CLAMP)
     np.testing.assert_array_equal(to_host(dst, np.float32), ref)
+    refb = np.zeros_like(big)
+    ob.local(big, refb, 5, 5, weights=w, boundary=capi.CLAMP)
+    np.testing.assert_array_equal(to_host(dbig, np.float32), refb)
