@@ -107,6 +107,8 @@ class TuneEntry(ctypes.Structure):
                 ("param", c_int32), ("ms", c_float)]
 
 
+NOALLOC_SCRATCH_BYTES = 16 * 1024 * 1024   # HIPACC_NOALLOC_SCRATCH_BYTES
+
 VAR_AUTO, VAR_TILE, VAR_STREAM, VAR_GENERIC, VAR_WIDE, VAR_RING = range(6)
 VARIANT_NAMES = {VAR_TILE: "tile", VAR_STREAM: "stream", VAR_GENERIC: "generic", VAR_WIDE: "wide",
                  VAR_RING: "ring"}
@@ -138,6 +140,7 @@ SIGNATURES = {
                                                 c_void_p, c_void_p]),
     "hipacc_hip_local_grad": (c_int32, [POINTER(LocalGradDesc), c_void_p]),
     "hipacc_hip_local_grad_variant": (c_int32, [POINTER(LocalGradDesc), c_int32, c_void_p]),
+    "hipacc_hip_local_grad_noalloc": (c_int32, [POINTER(LocalGradDesc), c_void_p]),
     "hipacc_hip_bilateral_train": (c_int32, [POINTER(BilateralGradDesc), c_void_p]),
     "hipacc_hip_bilateral_grad": (c_int32, [POINTER(BilateralGradDesc), c_void_p]),
     "hipacc_hip_alloc2d": (c_int32, [POINTER(c_void_p), POINTER(c_int32), c_int32, c_int32, c_int32,

@@ -51,9 +51,32 @@ struct DeviceState {
   void* red_ws = nullptr;
   void* grad_ws = nullptr;   // adjoint workspace, allocated / grown on demand
   size_t grad_bytes = 0;
+  void* scratch_ptr = nullptr;   // this device's copy of `scratch`
 };
 static DeviceState g_dev[kMaxDevices];
 
+// Workspace of the allocation-free adjoint (hipacc_hip_local_grad_noalloc):
+// a static device array, one copy per device, alive until the process ends.
+// Host code never passes `scratch` itself to a kernel (on the host that name
+// is only a stand-in); it uses the device address that hipGetSymbolAddress
+// returns for the current device, kept in DeviceState::scratch_ptr.
+static __device__ __align__(256) unsigned char scratch[HIPACC_NOALLOC_SCRATCH_BYTES];
+
+// The current device's address of `scratch`, looked up once per device
+// (with that device current).
+static int scratch_address(int dev, void** p) {
+  DeviceState& d = g_dev[dev];
+  std::lock_guard<std::mutex> lk(d.mu);
+  if (!d.scratch_ptr) {
+    void* q = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_status(e, "hipGetSymbolAddress(scratch)");
+    if (!q) return set_error(HIPACC_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    d.scratch_ptr = q;
+  }
+  *p = d.scratch_ptr;
+  return HIPACC_OK;
+}
 
 static int current_device(int* dev) {
   hipError_t e = hipGetDevice(dev);
@@ -307,6 +330,7 @@ int hipacc_hip_init(int device) {
   void* ws = nullptr;
   int st = ensure_workspace(device, &ws);
   if (st) return st;
+  if ((st = scratch_address(device, &ws))) return st;
   g_dev[device].ready = true;
   return HIPACC_OK;
 }
@@ -923,6 +947,27 @@ int hipacc_hip_local_grad_variant(const hipacc_local_grad_desc* d, int32_t varia
   return run_local_grad(d, variant, ws, true, static_cast<hipStream_t>(stream));
 }
 
+int hipacc_hip_local_grad_noalloc(const hipacc_local_grad_desc* d, void* stream) {
+  int st = check_local_grad(d);
+  if (st) return st;
+  if (d->workspace)
+    return set_error(HIPACC_ERR_INVALID_ARG,
+                     "local_grad_noalloc: desc->workspace must be NULL (the library's static "
+                     "scratch is the workspace)");
+  if (!d->grad_in.data && !d->grad_weights_dev) return HIPACC_OK;
+  const size_t need = conv_grad_ws(d).total;
+  if (need > (size_t)HIPACC_NOALLOC_SCRATCH_BYTES)
+    return set_error(HIPACC_ERR_INVALID_ARG,
+                     "local_grad_noalloc: needs %zu bytes of workspace, more than the %d-byte "
+                     "static scratch (use hipacc_hip_local_grad)",
+                     need, (int)HIPACC_NOALLOC_SCRATCH_BYTES);
+  int dev = 0;
+  if ((st = current_device(&dev))) return st;
+  void* p = nullptr;
+  if ((st = scratch_address(dev, &p))) return st;
+  return run_local_grad(d, kVarAuto, static_cast<char*>(p), false,
+                        static_cast<hipStream_t>(stream));
+}
 
 }  // extern "C"
 

@@ -25,9 +25,10 @@
  *    workspace per (device, stream), so they are reentrant per stream.
  *    hipacc_hip_reduce_sync holds its device's lock until its result is on
  *    the host.  The other users of a per-device library workspace
- *    (hipacc_hip_reduce and the adjoints with a NULL workspace) take the
- *    lock at enqueue time only: concurrent calls of those on different
- *    streams must pass their own workspace.
+ *    (hipacc_hip_reduce and the adjoints with a NULL workspace, and
+ *    hipacc_hip_local_grad_noalloc) take the lock at enqueue time only:
+ *    concurrent calls of those on different streams must pass their own
+ *    workspace (the no-allocation form has none and must not overlap).
  */
 #ifndef HIPACC_HIP_H
 #define HIPACC_HIP_H
@@ -306,6 +307,20 @@ int hipacc_hip_local_grad(const hipacc_local_grad_desc* desc, void* stream);
 int hipacc_hip_local_grad_variant(const hipacc_local_grad_desc* desc, int32_t variant,
                                   void* stream);
 
+/* Allocation-free form of hipacc_hip_local_grad: the same kernels and
+ * results, with the workspace taken from a static device array of
+ * HIPACC_NOALLOC_SCRATCH_BYTES that the library holds on each device until
+ * the process ends.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight
+ * training or serving loops, harnesses that forbid hipMalloc.  It calls no
+ * allocator.  desc->workspace must be NULL.  A descriptor whose
+ * hipacc_hip_local_grad_workspace_bytes exceeds the array fails with
+ * HIPACC_ERR_INVALID_ARG before anything is launched.  Every call on a
+ * device shares that device's array: calls that overlap on one device
+ * (several streams or host threads) race.  Launches are not timed by
+ * hipacc_hip_set_timing (no events are created).                           */
+#define HIPACC_NOALLOC_SCRATCH_BYTES (16 * 1024 * 1024)
+int hipacc_hip_local_grad_noalloc(const hipacc_local_grad_desc* desc, void* stream);
 
 /* ---- bilateral filter with a learnable range sigma -------------------- */
 /* Training forward: hipacc_hip_bilateral (F32, whole images) that also
