@@ -322,6 +322,71 @@ def bench_config5(op, capi, dev, reps):
             "gbs_at_9B_per_level_px": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def profile_ops(op, capi, masks, dev, f32_in, f32_out):
+    """Launches each benchmarked kernel family 10 times on its benchmark
+    workload (for rocprofv3 kernel-trace and counter runs)."""
+    import torch
+    u8_in = [torch.empty((H, W), dtype=torch.uint8, device=dev) for _ in range(NSETS)]
+    u8_out = [torch.empty_like(t) for t in u8_in]
+    i16 = [torch.empty((H, W), dtype=torch.int16, device=dev) for _ in range(NSETS)]
+    for i, t in enumerate(u8_in):
+        op.fill_random(t, 2000 + i)
+    sob = masks.SOBEL_X
+    g13 = masks.gaussian_1d(13, 3.0)
+    ones7 = np.ones(7, np.int32)
+    bins = torch.zeros((NSETS, 256), dtype=torch.int32, device=dev)
+    res = torch.zeros(NSETS, dtype=torch.float32, device=dev)
+    fams = {
+        "sobel_u8_i16": [op.prepare_local(u8_in[i], i16[i], 3, 3, weights=sob,
+                                          domain=(sob != 0).astype(np.uint8), acc_float=False)
+                         for i in range(NSETS)],
+        "sep13_f32": [op.prepare_separable(f32_in[i], f32_out[i], g13, g13) for i in range(NSETS)],
+        "sep13_u8": [op.prepare_separable(u8_in[i], u8_out[i], g13, g13, cvt=capi.CVT_ROUND)
+                     for i in range(NSETS)],
+        "box7_u8_u32": [op.prepare_separable(u8_in[i], f32_out[i], ones7, ones7, acc_float=False,
+                                             dst_dtype=capi.U32) for i in range(NSETS)],
+        "reduce_sum_f32": [op.ReduceCall(f32_in[i], capi.SUM, res[i:i + 1]) for i in range(NSETS)],
+        "histogram256_u8": [op.HistogramCall(u8_in[i], 256, bins[i]) for i in range(NSETS)],
+    }
+    for calls in fams.values():
+        for i in range(10):
+            calls[i % NSETS].launch()
+    torch.cuda.synchronize()
+    del u8_in, u8_out, i16
+    # training step kernels at 16 x 1024^2 (9x9 conv, 7x7 bilateral, CLAMP)
+    B = 16
+    x = torch.rand((B, 1024, 1024), device=dev)
+    g = torch.randn((B, 1024, 1024), device=dev)
+    wd = torch.from_numpy((0.1 * np.random.default_rng(4006).standard_normal(81))
+                          .astype(np.float32)).to(dev)
+    y1, y2, nrm, g1, gx = (torch.empty_like(x) for _ in range(5))
+    gw = torch.empty(81, device=dev)
+    gs = torch.empty(1, device=dev)
+    step = [op.prepare_local(x, y1, 9, 9, weights_dev=wd),
+            op.prepare_bilateral_train(y1, y2, nrm, 7, 2.0, 0.1, boundary=capi.CLAMP),
+            op.prepare_bilateral_grad(y1, y2, nrm, g, g1, 7, 2.0, 0.1, grad_sigma=gs,
+                                      boundary=capi.CLAMP),
+            op.prepare_local_grad(x, g1, gx, 9, 9, weights_dev=wd, grad_weights=gw)]
+    for _ in range(10):
+        for c in step:
+            c.launch()
+    torch.cuda.synchronize()
+    del x, g, y1, y2, nrm, g1, gx
+    # one batched 64-image pyramid, 8 levels (graph replays)
+    shapes = pyramid_workload_shapes()
+    gauss = [[torch.empty((h >> l, w >> l), device=dev) for l in range(8)] for h, w in shapes]
+    lap = [[torch.empty((h >> l, w >> l), device=dev) for l in range(7)] for h, w in shapes]
+    for i in range(len(shapes)):
+        op.fill_random(gauss[i][0], 6000 + i)
+    plan = op.PyramidPlan(gauss, lap, tile_rows=16)
+    for _ in range(3):
+        plan.run()
+    torch.cuda.synchronize()
+    plan.close()
+    print("profile run: " + ", ".join(f"10 x {k}" for k in fams) +
+          ", 10 training steps 16x1024^2, 3 batched pyramids", flush=True)
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -369,6 +434,10 @@ def main() -> int:
     ap.add_argument("--profile", action="store_true",
                     help="launch the headline kernel 20 times and a 64 MiB copy 20 times, "
                          "for rocprofv3 counter runs; prints no result line")
+    ap.add_argument("--profile-ops", action="store_true",
+                    help="with --profile: also launch the other benchmarked kernels 10 times "
+                         "each (u8 ring, separable rings, reduction, histogram, bilateral, "
+                         "training backward, one batched pyramid)")
     ap.add_argument("--variant", default="ring:0",
                     help="headline kernel variant under --profile (name:param)")
     args = ap.parse_args()
@@ -403,6 +472,8 @@ def main() -> int:
             dst.copy_(src)
         torch.cuda.synchronize()
         print(f"profile run: 20 x {args.variant} 5x5 f32 4096^2, 20 x 64 MiB copy", flush=True)
+        if args.profile_ops:
+            profile_ops(op, capi, masks, dev, f32_in, f32_out)
         return 0
     vname = tuned(head)
     ms, samples = median_ms(head, args.steps, args.warmup)

@@ -46,18 +46,7 @@ struct RingGeom {
   int segs_e, seg_rows_e;    // edge strips
   int per_img;               // items (strip segments) per image
   int items;                 // per_img * batch
-  int xcd;                   // 1: blocks of one XCD take a contiguous run of items
 };
-
-// Item of block b when the blocks of each XCD (b % 8: the hardware deals
-// blocks to the 8 XCDs in turn) take a contiguous run of the n items --
-// consecutive segments of a strip and neighbouring strips -- so the halo
-// rows and the cache lines two strips share are fetched into that XCD's L2
-// once.  A bijection of [0, n) for any n.
-__device__ __forceinline__ int ring_xcd_item(int b, int n) {
-  const int q = n >> 3, r = n & 7, x = b & 7, k = b >> 3;
-  return x * q + (x < r ? x : r) + k;
-}
 
 typedef unsigned int ring_u4 __attribute__((ext_vector_type(4)));
 
@@ -403,8 +392,7 @@ __global__ __launch_bounds__(256) void ring_kernel(const LocalArgs a, const Ring
   float4* lds = lds_ring + 1;
   const Geom& g = a.g;
   if ((int)blockIdx.x >= rg.items) return;
-  const RingItem it =
-      ring_item(rg, g, rg.xcd ? ring_xcd_item(blockIdx.x, rg.items) : (int)blockIdx.x);
+  const RingItem it = ring_item(rg, g, blockIdx.x);
   const float* in = static_cast<const float*>(g.in) + (long long)it.b * g.in_plane;
   float* out = static_cast<float*>(g.out) + (long long)it.b * g.out_plane;
   const int lane = threadIdx.x & 63;
@@ -696,8 +684,7 @@ __global__ __launch_bounds__(256) void ring_sep_kernel(const SepArgs a, const Ri
   __shared__ __align__(16) float4 lds[3 * CH * 64];
   const Geom& g = a.g;
   if ((int)blockIdx.x >= rg.items) return;
-  const RingItem it =
-      ring_item(rg, g, rg.xcd ? ring_xcd_item(blockIdx.x, rg.items) : (int)blockIdx.x);
+  const RingItem it = ring_item(rg, g, blockIdx.x);
   const TI* in = static_cast<const TI*>(g.in) + (long long)it.b * g.in_plane;
   TO* out = static_cast<TO*>(g.out) + (long long)it.b * g.out_plane;
   const int lane = threadIdx.x & 63;

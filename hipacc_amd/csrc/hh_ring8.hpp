@@ -285,8 +285,7 @@ __global__ __launch_bounds__(256, 3) void ring8_kernel(const LocalArgs a, const 
   __shared__ __align__(16) uint4 lds[(ND + 2 > 3 ? ND + 2 : 3) * CH * 64];
   const Geom& g = a.g;
   if ((int)blockIdx.x >= rg.items) return;
-  const RingItem it =
-      ring_item(rg, g, rg.xcd ? ring_xcd_item(blockIdx.x, rg.items) : (int)blockIdx.x);
+  const RingItem it = ring_item(rg, g, blockIdx.x);
   const uint8_t* in = static_cast<const uint8_t*>(g.in) + (long long)it.b * g.in_plane;
   TO* out = static_cast<TO*>(g.out) + (long long)it.b * g.out_plane;
   constexpr int V = kRing8V, R = K / 2;

@@ -2,7 +2,6 @@
 // (hh_ring.hpp): f32 -> f32 weighted SUM, square masks 3x3 .. 9x9, and
 // separable f32 masks 3 .. 13 taps.
 #include <cmath>
-#include <cstdlib>
 
 #include "hh_ring.hpp"
 
@@ -53,11 +52,6 @@ static float ring_edge_cost(int k) { return k <= 5 ? 1.4f : 1.15f; }
 RingGeom ring_geom(const Geom& g, int in_vec_ok, int out_vec_ok, float edge_cost, int hl, int v,
                    bool balance, int chunk, int seg_rows, int slots) {
   RingGeom rg;
-  static const int xcd_env = [] {
-    const char* e = std::getenv("HIPACC_RING_XCD");
-    return e ? std::atoi(e) : 1;
-  }();
-  rg.xcd = xcd_env;
   int sl = 64 - 2 * hl;                 // stored lanes per strip
   rg.strips = ceil_div(g.is_w, sl * v);
   if (balance) sl = ceil_div(ceil_div(g.is_w, rg.strips), v);
