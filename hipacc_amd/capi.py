@@ -77,14 +77,15 @@ class BilateralGradDesc(ctypes.Structure):
                 ("grad_sigma_r_dev", c_void_p), ("workspace", c_void_p)]
 
 
-INTERP_NN, INTERP_LF = 1, 2
+INTERP_NN, INTERP_LF, INTERP_B5, INTERP_CF, INTERP_L3 = 1, 2, 3, 4, 5
 RESAMPLE_COPY, RESAMPLE_SUB, RESAMPLE_ADD = 0, 1, 2
 MAX_LEVELS = 16
 
 
 class ResampleDesc(ctypes.Structure):
     _fields_ = [("input", View), ("in2", View), ("output", View), ("interp", c_int32),
-                ("op", c_int32), ("boundary", c_int32), ("const_val", c_double)]
+                ("op", c_int32), ("boundary", c_int32), ("const_val", c_double),
+                ("xf", c_int32), ("yf", c_int32)]
 
 
 class PyramidImage(ctypes.Structure):

@@ -1183,11 +1183,12 @@ int hipacc_hip_resample(const hipacc_resample_desc* d, void* stream) {
   int st = check_view(d->in, "in");
   if (st) return st;
   if ((st = check_view(d->out, "out"))) return st;
-  if (d->in.dtype != HIPACC_F32 || d->out.dtype != HIPACC_F32)
-    return set_error(HIPACC_ERR_UNSUPPORTED, "resample: F32 images only");
+  if ((d->in.dtype != HIPACC_F32 && d->in.dtype != HIPACC_U8) || d->out.dtype != d->in.dtype)
+    return set_error(HIPACC_ERR_UNSUPPORTED,
+                     "resample: F32 or U8 images, the output of the input's type");
   if (batch_of(d->in) != 1 || batch_of(d->out) != 1)
     return set_error(HIPACC_ERR_UNSUPPORTED, "resample: single images only");
-  if (d->interp != HIPACC_INTERP_NN && d->interp != HIPACC_INTERP_LF)
+  if (d->interp < HIPACC_INTERP_NN || d->interp > HIPACC_INTERP_L3)
     return set_error(HIPACC_ERR_INVALID_ARG, "resample: interp %d", d->interp);
   if (d->op < HIPACC_RESAMPLE_COPY || d->op > HIPACC_RESAMPLE_ADD)
     return set_error(HIPACC_ERR_INVALID_ARG, "resample: op %d", d->op);
@@ -1195,15 +1196,20 @@ int hipacc_hip_resample(const hipacc_resample_desc* d, void* stream) {
     return set_error(HIPACC_ERR_INVALID_ARG, "resample: boundary %d", d->boundary);
   if (d->op != HIPACC_RESAMPLE_COPY) {
     if ((st = check_view(d->in2, "in2"))) return st;
-    if (d->in2.dtype != HIPACC_F32 || d->in2.width != d->out.width ||
-        d->in2.height != d->out.height || batch_of(d->in2) != 1)
-      return set_error(HIPACC_ERR_INVALID_ARG, "resample: in2 must be F32 of out's size");
+    if (d->in2.dtype != HIPACC_F32 || d->out.dtype != HIPACC_F32 ||
+        d->in2.width != d->out.width || d->in2.height != d->out.height || batch_of(d->in2) != 1)
+      return set_error(HIPACC_ERR_INVALID_ARG,
+                       "resample: SUB / ADD take F32 images, in2 of out's size");
   }
   ResampleArgs a;
   std::memset(&a, 0, sizeof(a));
-  a.in = static_cast<const float*>(d->in.data);
+  a.in = d->in.data;
   a.in2 = static_cast<const float*>(d->in2.data);
-  a.out = static_cast<float*>(d->out.data);
+  a.out = d->out.data;
+  a.in_dtype = d->in.dtype;
+  a.out_dtype = d->out.dtype;
+  a.xf = d->xf;
+  a.yf = d->yf;
   a.in_stride = d->in.stride;
   a.in2_stride = d->in2.stride;
   a.out_stride = d->out.stride;
@@ -1224,7 +1230,12 @@ int hipacc_hip_resample(const hipacc_resample_desc* d, void* stream) {
   a.interp = d->interp;
   a.op = d->op;
   a.boundary = d->boundary;
-  a.cval = (float)d->const_val;
+  {
+    float cf;
+    int ci;
+    const_views(d->in.dtype, d->const_val, &cf, &ci);
+    a.cval = cf;   // the constant in the pixel type, seen as float
+  }
   hipStream_t s = static_cast<hipStream_t>(stream);
   Timer tm(s);
   return tm.finish(launch_resample(a, s));

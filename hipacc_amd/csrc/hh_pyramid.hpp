@@ -32,10 +32,53 @@ HH_HD float interp_bilinear(float fx, float fy, float p00, float p10, float p01,
   return s1 + t11;
 }
 
+// Interpolation weights of the reference's Interpolation class
+// (dsl/image.hpp:316-382), in its operation order: binomial_5, Keys' cubic
+// convolution with a = -0.5 (bicubic_spline), and Lanczos-3 evaluated in
+// double and rounded once to float.
+HH_HD float interp_w_b5(float d) {
+  d = fabsf(d);
+  if (d < 0.5f) return 6.0f / 8.0f;
+  if (d < 1.0f) return 4.0f / 8.0f;
+  if (d < 1.5f) return 1.0f / 8.0f;
+  return 0.0f;
+}
+HH_HD float interp_w_cf(float d) {
+  const float a = -0.5f;
+  d = fabsf(d);
+  if (d < 1.0f) {
+    const float t3 = (a + 2.0f) * d * d * d;
+    const float t2 = (a + 3.0f) * d * d;
+    return (t3 - t2) + 1.0f;
+  }
+  if (d < 2.0f) {
+    const float t3 = a * d * d * d;
+    const float t2 = 5.0f * a * d * d;
+    const float t1 = 8.0f * a * d;
+    const float t0 = 4.0f * a;
+    return ((t3 - t2) + t1) - t0;
+  }
+  return 0.0f;
+}
+HH_HD float interp_w_l3(float d) {
+  const float l = 3.0f;
+  d = fabsf(d);
+  if (d == 0.0f) return 1.0f;
+  if (d < l) {
+    const double pi = 3.141592653589793;   // atan(1.0) * 4, as the reference
+    const double num = (double)l * (sin(pi * d / l) * sin(pi * d));
+    const double den = pi * pi * d * d;
+    return (float)(num / den);
+  }
+  return 0.0f;
+}
+
 struct ResampleArgs {
-  const float* in;
+  const void* in;              // in_dtype pixels (F32 or U8)
   const float* in2;
-  float* out;
+  void* out;                   // out_dtype pixels (the input's type)
+  int in_dtype, out_dtype;
+  int xf, yf;                  // accessor call offsets in(xf, yf)
   int in_stride, in2_stride, out_stride;
   int in_imgw, in_imgh;
   int lo_x, lo_y, hi_x, hi_y;      // accessor window

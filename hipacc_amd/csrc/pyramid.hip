@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "hh_pyramid.hpp"
@@ -14,6 +15,7 @@ namespace hh {
 // ---------------------------------------------------------------------------
 // Resampling point operator
 // ---------------------------------------------------------------------------
+template <typename T>
 __device__ __forceinline__ float rs_read(const ResampleArgs& a, int x, int y) {
   if (a.boundary == HIPACC_BOUNDARY_CONSTANT) {
     if (x < a.lo_x || x >= a.hi_x || y < a.lo_y || y >= a.hi_y) return a.cval;
@@ -21,37 +23,122 @@ __device__ __forceinline__ float rs_read(const ResampleArgs& a, int x, int y) {
     x = bh_map(a.boundary, x, a.lo_x, a.hi_x, a.in_imgw);
     y = bh_map(a.boundary, y, a.lo_y, a.hi_y, a.in_imgh);
   }
-  return a.in[(long long)y * a.in_stride + x];
+  return (float)static_cast<const T*>(a.in)[(long long)y * a.in_stride + x];
 }
 
-__global__ __launch_bounds__(256) void resample_kernel(const ResampleArgs a) {
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (x >= a.w || y >= a.h) return;
-  const float mx = interp_map(a.sx, x, a.lo_x);
-  const float my = interp_map(a.sy, y, a.lo_y);
-  float v;
-  if (a.interp == HIPACC_INTERP_NN) {
-    v = rs_read(a, (int)mx, (int)my);
+// Sum of NX taps of source row yr starting at column x0, weights w(fx + k)
+// (k = 0 .. NX-1, with the L3 quirk of the second row: its fifth tap takes
+// the sixth tap's weight), left to right with one rounding per product and
+// per add.
+template <typename T, int MODE>
+__device__ __forceinline__ float rs_row(const ResampleArgs& a, int x0, int yr, float fx, int row) {
+  constexpr int NX = MODE == HIPACC_INTERP_L3 ? 6 : 4;
+  float acc = 0.0f;
+#pragma unroll
+  for (int k = 0; k < NX; ++k) {
+    float w;
+    if constexpr (MODE == HIPACC_INTERP_B5) w = interp_w_b5(fx - (float)k);
+    else if constexpr (MODE == HIPACC_INTERP_CF) w = interp_w_cf(fx + (float)k);
+    else w = interp_w_l3(fx + (float)((row == 1 && k == 4) ? 5 : k));
+    const float t = rs_read<T>(a, x0 + k, yr) * w;
+    acc = k == 0 ? t : acc + t;
+  }
+  return acc;
+}
+
+// Interpolation::interpolate (dsl/image.hpp:390-532) at iteration-space
+// pixel (x, y) read at (x + xf, y + yf).
+template <typename T, int MODE>
+__device__ __forceinline__ float rs_value(const ResampleArgs& a, int x, int y) {
+  const float mx = interp_map(a.sx, x + a.xf, a.lo_x);
+  const float my = interp_map(a.sy, y + a.yf, a.lo_y);
+  if constexpr (MODE == HIPACC_INTERP_NN) {
+    return rs_read<T>(a, (int)mx, (int)my);
+  } else if constexpr (MODE == HIPACC_INTERP_LF) {
+    int xi, yi;
+    float fx, fy;
+    interp_lf(mx, xi, fx);
+    interp_lf(my, yi, fy);
+    return interp_bilinear(fx, fy, rs_read<T>(a, xi, yi), rs_read<T>(a, xi + 1, yi),
+                           rs_read<T>(a, xi, yi + 1), rs_read<T>(a, xi + 1, yi + 1));
   } else {
     int xi, yi;
     float fx, fy;
     interp_lf(mx, xi, fx);
     interp_lf(my, yi, fy);
-    v = interp_bilinear(fx, fy, rs_read(a, xi, yi), rs_read(a, xi + 1, yi),
-                        rs_read(a, xi, yi + 1), rs_read(a, xi + 1, yi + 1));
+    // B5: frac + 0.5, taps x_int + k, weight b5(frac - k);
+    // CF: taps x_int - 1 + k, weight cf((frac - 1) + k);
+    // L3: taps x_int - 2 + k, rows y_int - 1 + k, weight l3((frac - 2) + k)
+    constexpr int NY = MODE == HIPACC_INTERP_L3 ? 6 : 4;
+    float bx, by;
+    int x0, y0;
+    if constexpr (MODE == HIPACC_INTERP_B5) {
+      bx = fx + 0.5f;
+      by = fy + 0.5f;
+      x0 = xi;
+      y0 = yi;
+    } else if constexpr (MODE == HIPACC_INTERP_CF) {
+      bx = fx - 1.0f;
+      by = fy - 1.0f;
+      x0 = xi - 1;
+      y0 = yi - 1;
+    } else {
+      bx = fx - 2.0f;
+      by = fy - 2.0f;
+      x0 = xi - 2;
+      y0 = yi - 1;
+    }
+    float v = 0.0f;
+#pragma unroll
+    for (int r = 0; r < NY; ++r) {
+      float wy;
+      if constexpr (MODE == HIPACC_INTERP_B5) wy = interp_w_b5(by - (float)r);
+      else if constexpr (MODE == HIPACC_INTERP_CF) wy = interp_w_cf(by + (float)r);
+      else wy = interp_w_l3(by + (float)r);
+      const float t = rs_row<T, MODE>(a, x0, y0 + r, bx, r) * wy;
+      v = r == 0 ? t : v + t;
+    }
+    return v;
   }
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void resample_kernel(const ResampleArgs a) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= a.w || y >= a.h) return;
+  float v = rs_value<T, MODE>(a, x, y);
   if (a.op != HIPACC_RESAMPLE_COPY) {
     const float b = a.in2[(long long)(y + a.in2_oy) * a.in2_stride + x + a.in2_ox];
     v = a.op == HIPACC_RESAMPLE_SUB ? b - v : v + b;
   }
-  a.out[(long long)(y + a.out_oy) * a.out_stride + x + a.out_ox] = v;
+  const long long o = (long long)(y + a.out_oy) * a.out_stride + x + a.out_ox;
+  if constexpr (std::is_same<T, float>::value) static_cast<float*>(a.out)[o] = v;
+  else store_f(a.out, a.out_dtype, o, v, HIPACC_CVT_TRUNC, 0.0f);
+}
+
+template <typename T>
+static int launch_resample_t(const ResampleArgs& a, dim3 grid, hipStream_t s) {
+  switch (a.interp) {
+    case HIPACC_INTERP_NN:
+      hipLaunchKernelGGL((resample_kernel<T, HIPACC_INTERP_NN>), grid, dim3(256), 0, s, a); break;
+    case HIPACC_INTERP_LF:
+      hipLaunchKernelGGL((resample_kernel<T, HIPACC_INTERP_LF>), grid, dim3(256), 0, s, a); break;
+    case HIPACC_INTERP_B5:
+      hipLaunchKernelGGL((resample_kernel<T, HIPACC_INTERP_B5>), grid, dim3(256), 0, s, a); break;
+    case HIPACC_INTERP_CF:
+      hipLaunchKernelGGL((resample_kernel<T, HIPACC_INTERP_CF>), grid, dim3(256), 0, s, a); break;
+    case HIPACC_INTERP_L3:
+      hipLaunchKernelGGL((resample_kernel<T, HIPACC_INTERP_L3>), grid, dim3(256), 0, s, a); break;
+    default: return set_error(HIPACC_ERR_INVALID_ARG, "resample: interp %d", a.interp);
+  }
+  return hip_status(hipGetLastError(), "resample_kernel launch");
 }
 
 int launch_resample(const ResampleArgs& a, hipStream_t s) {
   dim3 grid(ceil_div(a.w, 64), ceil_div(a.h, 4));
-  hipLaunchKernelGGL(resample_kernel, grid, dim3(256), 0, s, a);
-  return hip_status(hipGetLastError(), "resample_kernel launch");
+  return a.in_dtype == HIPACC_U8 ? launch_resample_t<uint8_t>(a, grid, s)
+                                 : launch_resample_t<float>(a, grid, s);
 }
 
 // ---------------------------------------------------------------------------

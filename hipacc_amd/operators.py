@@ -331,15 +331,18 @@ def prepare_bilateral_grad(src: torch.Tensor, dst: torch.Tensor, norm: torch.Ten
 def prepare_resample(src: torch.Tensor, dst: torch.Tensor, interp: int, *,
                      op: int = capi.RESAMPLE_COPY, src2: Optional[torch.Tensor] = None,
                      boundary: int = capi.CLAMP, const_val: float = 0.0,
-                     src_window: Window = None, dst_window: Window = None) -> Call:
-    """Point operator reading `src` through Interpolate::NN / LF; op SUB
-    gives src2 - v, ADD v + src2."""
+                     src_window: Window = None, dst_window: Window = None, xf: int = 0,
+                     yf: int = 0) -> Call:
+    """Point operator reading `src` (f32 or u8; dst of the same type)
+    through Interpolate::NN / LF / B5 / CF / L3 at the iteration-space pixel
+    offset (xf, yf); op SUB gives src2 - v, ADD v + src2 (f32)."""
     d = capi.ResampleDesc()
     d.input = view(src, src_window)
     if src2 is not None:
         d.in2 = view(src2)
     d.output = view(dst, dst_window)
     d.interp, d.op, d.boundary, d.const_val = interp, op, boundary, const_val
+    d.xf, d.yf = xf, yf
     return Call("hipacc_hip_resample", d, [src2])
 
 

@@ -370,7 +370,8 @@ int hipacc_hip_bilateral_grad(const hipacc_bilateral_grad_desc* desc, void* stre
  * These are the Subsample, DifferenceOfGaussian and Restore kernels of the
  * pyramid sample (samples-public/5_Other/Gaussian_Laplacian_Pyramid/src/
  * main.cpp:67-117).  in_bh uses `boundary` over the accessor window.        */
-typedef enum { HIPACC_INTERP_NN = 1, HIPACC_INTERP_LF = 2 } hipacc_interp;
+typedef enum { HIPACC_INTERP_NN = 1, HIPACC_INTERP_LF = 2, HIPACC_INTERP_B5 = 3,
+               HIPACC_INTERP_CF = 4, HIPACC_INTERP_L3 = 5 } hipacc_interp;
 typedef enum { HIPACC_RESAMPLE_COPY = 0, HIPACC_RESAMPLE_SUB = 1,
                HIPACC_RESAMPLE_ADD = 2 } hipacc_resample_op;
 typedef struct hipacc_resample_desc {
@@ -381,6 +382,8 @@ typedef struct hipacc_resample_desc {
   int32_t op;                /* hipacc_resample_op                          */
   int32_t boundary;
   double  const_val;
+  int32_t xf, yf;            /* the accessor call in(xf, yf): pixel (x, y) of
+                                the iteration space maps x + xf, y + yf     */
 } hipacc_resample_desc;
 int hipacc_hip_resample(const hipacc_resample_desc* desc, void* stream);
 

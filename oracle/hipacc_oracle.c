@@ -632,40 +632,135 @@ ORC_EXPORT int oracle_bilateral_f64(const hipacc_bilateral_desc* d, double* out_
  * Interpolate::NN / LF, in float with the DSL's evaluation order (this file
  * is compiled without contraction), reading through the accessor's boundary
  * handling; then COPY / SUB (in2 - v) / ADD (v + in2). -------------------- */
+/* Interpolation weights of dsl/image.hpp:316-382 (binomial_5,
+ * bicubic_spline with a = -0.5, lanczos with l = 3, the last evaluated in
+ * double and rounded once to float), in the source's operation order.      */
+static float w_b5(float d) {
+  d = fabsf(d);
+  if (d < 0.5f) return 6.0f / 8.0f;
+  if (d < 1.0f) return 4.0f / 8.0f;
+  if (d < 1.5f) return 1.0f / 8.0f;
+  return 0.0f;
+}
+static float w_cf(float d) {
+  const float a = -0.5f;
+  d = fabsf(d);
+  if (d < 1.0f) {
+    const float t3 = (a + 2.0f) * d * d * d;
+    const float t2 = (a + 3.0f) * d * d;
+    return (t3 - t2) + 1.0f;
+  }
+  if (d < 2.0f) {
+    const float t3 = a * d * d * d;
+    const float t2 = 5.0f * a * d * d;
+    const float t1 = 8.0f * a * d;
+    const float t0 = 4.0f * a;
+    return ((t3 - t2) + t1) - t0;
+  }
+  return 0.0f;
+}
+static float w_l3(float d) {
+  const float l = 3.0f;
+  d = fabsf(d);
+  if (d == 0.0f) return 1.0f;
+  if (d < l) {
+    const double pi = atan(1.0) * 4;
+    const double num = (double)l * (sin(pi * d / l) * sin(pi * d));
+    const double den = pi * pi * d * d;
+    return (float)(num / den);
+  }
+  return 0.0f;
+}
+
+/* Interpolation::interpolate (dsl/image.hpp:390-532) for the modes NN, LF,
+ * B5, CF, L3 of an accessor window read with its boundary condition; the
+ * iteration-space pixel x is read at x + xf (the accessor call in(xf, yf)).
+ * Sums run left to right with a rounding per product and per add: rows of
+ * taps first (y0 .. y3 / y5), then the row sums weighted by the y weights.
+ * L3 keeps the reference's row weights lanczos(y_frac - 2 + k) for rows
+ * y_int - 1 + k and its second row's fifth tap weight lanczos(x_frac - 2 + 5)
+ * (dsl/image.hpp:493; the generated CUDA code has the same,
+ * runtime/hipacc_cu_interpolate.hpp:491).                                   */
+static float rs_px(const hipacc_resample_desc* d, int x, int y, float cf) {
+  long long idx = 0;
+  return bh_tap(&d->in, d->boundary, x, y, &idx) ? px_f(d->in.data, d->in.dtype, idx) : cf;
+}
 ORC_EXPORT int oracle_resample(const hipacc_resample_desc* d) {
   const hipacc_view* in = &d->in;
   const hipacc_view* out = &d->out;
   const float stride_x = in->width / (float)out->width;
   const float stride_y = in->height / (float)out->height;
-  const float* ip = (const float*)in->data;
-  const float cf = (float)d->const_val;
+  const float cf = const_f(in->dtype, d->const_val);
   for (int y = 0; y < out->height; ++y)
     for (int x = 0; x < out->width; ++x) {
-      const float x_mapped = in->offset_x + stride_x / 2 + stride_x * x;
-      const float y_mapped = in->offset_y + stride_y / 2 + stride_y * y;
-      float v;
-      long long idx = 0;
-      if (d->interp == HIPACC_INTERP_NN) {
-        v = bh_tap(in, d->boundary, (int)x_mapped, (int)y_mapped, &idx) ? ip[idx] : cf;
-      } else {
-        float xb = x_mapped - 0.5f, yb = y_mapped - 0.5f;
-        if (xb < 0.0f) xb = 0.0f;
-        if (yb < 0.0f) yb = 0.0f;
-        const int xi = (int)xb, yi = (int)yb;
-        const float xf = xb - xi, yf = yb - yi;
-        const float p00 = bh_tap(in, d->boundary, xi, yi, &idx) ? ip[idx] : cf;
-        const float p10 = bh_tap(in, d->boundary, xi + 1, yi, &idx) ? ip[idx] : cf;
-        const float p01 = bh_tap(in, d->boundary, xi, yi + 1, &idx) ? ip[idx] : cf;
-        const float p11 = bh_tap(in, d->boundary, xi + 1, yi + 1, &idx) ? ip[idx] : cf;
-        v = (1.0f - xf) * (1.0f - yf) * p00 + xf * (1.0f - yf) * p10 +
-            (1.0f - xf) * yf * p01 + xf * yf * p11;
+      const float bx = in->offset_x + stride_x / 2, by = in->offset_y + stride_y / 2;
+      const float x_mapped = bx + stride_x * (float)(x + d->xf);
+      const float y_mapped = by + stride_y * (float)(y + d->yf);
+      float xb = x_mapped - 0.5f, yb = y_mapped - 0.5f;
+      if (xb < 0.0f) xb = 0.0f;
+      if (yb < 0.0f) yb = 0.0f;
+      const int xi = (int)xb, yi = (int)yb;
+      float xf = xb - xi, yf = yb - yi;
+      float v = 0.0f;
+      switch (d->interp) {
+        case HIPACC_INTERP_NN:
+          v = rs_px(d, (int)x_mapped, (int)y_mapped, cf);
+          break;
+        case HIPACC_INTERP_LF: {
+          const float p00 = rs_px(d, xi, yi, cf), p10 = rs_px(d, xi + 1, yi, cf);
+          const float p01 = rs_px(d, xi, yi + 1, cf), p11 = rs_px(d, xi + 1, yi + 1, cf);
+          v = (1.0f - xf) * (1.0f - yf) * p00 + xf * (1.0f - yf) * p10 +
+              (1.0f - xf) * yf * p01 + xf * yf * p11;
+          break;
+        }
+        case HIPACC_INTERP_B5: {
+          xf += 0.5f;
+          yf += 0.5f;
+          float r[4];
+          for (int k = 0; k < 4; ++k) {
+            float a = rs_px(d, xi, yi + k, cf) * w_b5(xf - 0);
+            for (int j = 1; j < 4; ++j) a = a + rs_px(d, xi + j, yi + k, cf) * w_b5(xf - (float)j);
+            r[k] = a;
+          }
+          v = r[0] * w_b5(yf - 0);
+          for (int k = 1; k < 4; ++k) v = v + r[k] * w_b5(yf - (float)k);
+          break;
+        }
+        case HIPACC_INTERP_CF: {
+          float r[4];
+          for (int k = 0; k < 4; ++k) {
+            float a = rs_px(d, xi - 1, yi - 1 + k, cf) * w_cf((xf - 1.0f) + 0.0f);
+            for (int j = 1; j < 4; ++j)
+              a = a + rs_px(d, xi - 1 + j, yi - 1 + k, cf) * w_cf((xf - 1.0f) + (float)j);
+            r[k] = a;
+          }
+          v = r[0] * w_cf((yf - 1.0f) + 0.0f);
+          for (int k = 1; k < 4; ++k) v = v + r[k] * w_cf((yf - 1.0f) + (float)k);
+          break;
+        }
+        case HIPACC_INTERP_L3: {
+          float r[6];
+          for (int k = 0; k < 6; ++k) {
+            float a = rs_px(d, xi - 2, yi - 1 + k, cf) * w_l3((xf - 2.0f) + 0.0f);
+            for (int j = 1; j < 6; ++j) {
+              const int jw = (k == 1 && j == 4) ? 5 : j;
+              a = a + rs_px(d, xi - 2 + j, yi - 1 + k, cf) * w_l3((xf - 2.0f) + (float)jw);
+            }
+            r[k] = a;
+          }
+          v = r[0] * w_l3((yf - 2.0f) + 0.0f);
+          for (int k = 1; k < 6; ++k) v = v + r[k] * w_l3((yf - 2.0f) + (float)k);
+          break;
+        }
+        default: return 1;
       }
       if (d->op != HIPACC_RESAMPLE_COPY) {
-        const float b = ((const float*)d->in2.data)[(long long)(y + d->in2.offset_y) * d->in2.stride +
-                                                    x + d->in2.offset_x];
+        const float b = px_f(d->in2.data, d->in2.dtype,
+                             (long long)(y + d->in2.offset_y) * d->in2.stride + x + d->in2.offset_x);
         v = d->op == HIPACC_RESAMPLE_SUB ? b - v : v + b;
       }
-      ((float*)out->data)[(long long)(y + out->offset_y) * out->stride + x + out->offset_x] = v;
+      px_put_f(out->data, out->dtype, (long long)(y + out->offset_y) * out->stride + x + out->offset_x,
+               v, HIPACC_CVT_TRUNC, 0.0f);
     }
   return 0;
 }

@@ -24,6 +24,11 @@
 //     nn_f32    <dst_w> <dst_h>           point over a dst_w x dst_h space reading
 //                                         the source through Interpolate::NN
 //     lf_f32    <dst_w> <dst_h>           same through Interpolate::LF
+//     interp_f32 / interp_u8 <imode 1..5> <dst_w> <dst_h> <xf> <yf> <cx> <cy> <cw> <ch>
+//                                         output() = in(xf, yf) through Interpolate
+//                                         NN/LF/B5/CF/L3, the accessor a crop
+//                                         (cx, cy, cw, ch; cw = 0: the whole
+//                                         image) with the boundary condition
 #include "hipacc.hpp"
 
 #include <cmath>
@@ -175,6 +180,33 @@ class Resample : public Kernel<float> {
   Resample(IterationSpace<float>& is, Accessor<float>& in) : Kernel(is), in(in) { add_accessor(&in); }
   void kernel() { output() = in(); }
 };
+
+template <typename T> class ResampleAt : public Kernel<T> {
+  Accessor<T>& in;
+  const int xf, yf;
+ public:
+  ResampleAt(IterationSpace<T>& is, Accessor<T>& in, int xf, int yf)
+      : Kernel<T>(is), in(in), xf(xf), yf(yf) { this->add_accessor(&in); }
+  void kernel() { this->output() = in(xf, yf); }
+};
+
+template <typename T>
+static int run_interp(int w, int h, uint64_t seed, int imode, int dw, int dh, int xf, int yf,
+                      int cx, int cy, int cw, int ch, const char* path) {
+  auto px = pixels<T>(w, h, seed);
+  Image<T> in(w, h, px.data()), out(dw, dh);
+  // CONSTANT needs a constant (the DSL reads T{} whatever value is given)
+  BoundaryCondition<T> bc = g_bmode == Boundary::CONSTANT
+                                ? BoundaryCondition<T>(in, 1, 1, g_bmode, T(0))
+                                : BoundaryCondition<T>(in, 1, 1, g_bmode);
+  if (cw <= 0) { cx = 0; cy = 0; cw = w; ch = h; }
+  Accessor<T> acc(bc, cw, ch, cx, cy, (Interpolate)imode);
+  IterationSpace<T> is(out);
+  ResampleAt<T> k(is, acc, xf, yf);
+  k.execute();
+  write_out(path, out.data(), (size_t)dw * dh * sizeof(T));
+  return 0;
+}
 
 // ---- drivers ----------------------------------------------------------------
 template <int K> static int run_conv_u8(int w, int h, uint64_t seed, double sigma, const char* path) {
@@ -379,6 +411,13 @@ int main(int argc, char** argv) {
     k.execute();
     write_out(path, out.data(), (size_t)dw * dh * 4);
     return 0;
+  }
+  if (op == "interp_f32" || op == "interp_u8") {
+    int v[9];
+    for (int i = 0; i < 9; ++i) v[i] = std::atoi(arg(i));
+    if (op == "interp_f32")
+      return run_interp<float>(w, h, seed, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], path);
+    return run_interp<uchar>(w, h, seed, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], path);
   }
   std::fprintf(stderr, "unknown op/size: %s\n", op.c_str());
   return 2;

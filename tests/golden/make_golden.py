@@ -73,6 +73,22 @@ def cases():
     for (w, h, dw, dh) in ((37, 29, 18, 14), (64, 48, 32, 24), (18, 14, 37, 29), (32, 24, 64, 48)):
         add("nn_f32", w, h, "CLAMP", [dw, dh], "float32", [dh, dw])
         add("lf_f32", w, h, "CLAMP", [dw, dh], "float32", [dh, dw])
+    # the whole interpolation set (NN, LF, B5, CF, L3) on float and uchar
+    # accessors: down / up sampling, a cropped accessor window with call
+    # offsets in(xf, yf), each boundary mode (REPEAT on a whole image: on a
+    # crop the DSL's single shift by lower + upper differs from the generated
+    # code's periodic wrap, which the library follows)
+    geoms = (  # w, h, dst_w, dst_h, xf, yf, crop (x, y, w, h; 0 = whole), mode
+        (37, 29, 18, 14, 0, 0, (0, 0, 0, 0), "CLAMP"),
+        (18, 14, 37, 29, 0, 0, (0, 0, 0, 0), "MIRROR"),
+        (40, 32, 30, 25, 1, -2, (5, 3, 20, 17), "CLAMP"),
+        (29, 35, 45, 17, 2, 1, (0, 0, 0, 0), "REPEAT"),
+        (33, 21, 50, 40, -1, 1, (0, 0, 0, 0), "CONSTANT"),
+    )
+    for imode in (1, 2, 3, 4, 5):
+        for dt, op in (("float32", "interp_f32"), ("uint8", "interp_u8")):
+            for (w, h, dw, dh, xf, yf, crop, m) in geoms:
+                add(op, w, h, m, [imode, dw, dh, xf, yf, *crop], dt, [dh, dw])
     return out
 
 

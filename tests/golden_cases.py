@@ -28,7 +28,7 @@ def input_for(c):
     """The SplitMix64 input(s) the reference driver generated for a case."""
     w, h, seed = c["w"], c["h"], c["seed"]
     op = c["op"]
-    if op in ("conv_u8", "sobel_u8", "box_u8", "min_u8", "red_u8", "hist_u8"):
+    if op in ("conv_u8", "sobel_u8", "box_u8", "min_u8", "red_u8", "hist_u8", "interp_u8"):
         return ob.random_image((h, w), np.uint8, seed)
     if op == "sub_i8":
         return (ob.random_image((h, w), np.int8, seed), ob.random_image((h, w), np.int8, seed + 1))
@@ -61,3 +61,14 @@ def local_spec(c):
         return dict(mask_w=k, mask_h=k, acc_float=True, reduce=capi.MAX, boundary=b,
                     out_dtype=np.float32)
     return None
+
+
+def interp_spec(c):
+    """(interp, dst shape, kwargs) of a resample call reproducing an
+    interp_f32 / interp_u8 case."""
+    a = [int(v) for v in c["args"]]
+    imode, dw, dh, xf, yf, cx, cy, cw, ch = a
+    kw = dict(boundary=c["boundary"], xf=xf, yf=yf)
+    if cw > 0:
+        kw["src_window"] = (cx, cy, cw, ch)
+    return imode, (dh, dw), kw

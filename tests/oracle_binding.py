@@ -256,14 +256,16 @@ def bilateral_grad(src, grad_out, ksize, sigma_d, sigma_r, *, spatial=None, boun
 
 # ---- resampling and pyramids ---------------------------------------------------
 def resample(src, dst, interp, *, op=capi.RESAMPLE_COPY, src2=None, boundary=capi.CLAMP,
-             const_val=0.0, src_window: Window = None):
+             const_val=0.0, src_window: Window = None, dst_window: Window = None, xf=0, yf=0):
     d = capi.ResampleDesc()
     d.input = hview(src, src_window)
     if src2 is not None:
         d.in2 = hview(src2)
-    d.output = hview(dst)
+    d.output = hview(dst, dst_window)
     d.interp, d.op, d.boundary, d.const_val = interp, op, boundary, const_val
-    lib().oracle_resample(ctypes.byref(d))
+    d.xf, d.yf = xf, yf
+    if lib().oracle_resample(ctypes.byref(d)):
+        raise ValueError(f"oracle_resample: interp {interp}")
 
 
 BINOMIAL5 = (np.outer([1, 4, 6, 4, 1], [1, 4, 6, 4, 1]) / 256.0).astype(np.float32).ravel()

@@ -94,3 +94,67 @@ def test_plan_tiles_rejects_bad_sizes():
     from hipacc_amd import operators as op
     with pytest.raises(capi.HipaccError):
         op.pyramid_plan_tiles([(3, 300)], 4)   # no pixel left at level 2
+
+
+# ---- the interpolation set (NN / LF / B5 / CF / L3) ---------------------------
+def test_interpolation_closed_forms():
+    """B5 of a constant image is c * Sx * Sy, S = 11/8 (1 where frac is 0
+    or 0.5); CF with the reference's tap weights scales a constant by the
+    sums of its x and y weights; NN of an up-sampled image repeats source
+    pixels."""
+    c = np.full((20, 24), 0.75, np.float32)
+    out = np.zeros((33, 41), np.float32)
+    ob.resample(c, out, capi.INTERP_B5, boundary=capi.CLAMP)
+
+    def tap_sum(n_src, n_dst):
+        # the binomial weights at distances frac + 0.5 - k sum to 11/8, or to
+        # 1 where a distance hits a step of the weight (frac 0 or 0.5)
+        s = np.float32(n_src) / np.float32(n_dst)
+        xm = (s / np.float32(2)) + s * np.arange(n_dst, dtype=np.float32)
+        xb = np.maximum(xm - np.float32(0.5), np.float32(0))
+        f = xb - np.floor(xb)
+        return np.where((f == 0) | (f == 0.5), 1.0, 11 / 8)
+    want = 0.75 * np.outer(tap_sum(20, 33), tap_sum(24, 41))
+    np.testing.assert_allclose(out, want, rtol=1e-6)
+    # CF: pixel x_int - 1 + k takes Keys' weight of (frac - 1) + k, as the
+    # reference does (dsl/image.hpp:455-478 and the generated CUDA code):
+    # a constant image gives c * Sx * Sy with S = sum_k w(frac - 1 + k),
+    # which is not 1 (the weights are mirrored about the pixel)
+    ob.resample(c, out, capi.INTERP_CF, boundary=capi.CLAMP)
+
+    def keys(d):
+        d = np.abs(d)
+        return np.where(d < 1, 1.5 * d ** 3 - 2.5 * d ** 2 + 1,
+                        np.where(d < 2, -0.5 * d ** 3 + 2.5 * d ** 2 - 4 * d + 2, 0.0))
+
+    def cf_sum(n_src, n_dst):
+        s = np.float32(n_src) / np.float32(n_dst)
+        xm = (s / np.float32(2)) + s * np.arange(n_dst, dtype=np.float32)
+        xb = np.maximum(xm - np.float32(0.5), np.float32(0)).astype(np.float64)
+        f = xb - np.floor(xb)
+        return sum(keys(f - 1 + k) for k in range(4))
+    want = 0.75 * np.outer(cf_sum(20, 33), cf_sum(24, 41))
+    np.testing.assert_allclose(out, want, rtol=1e-5)
+    assert np.abs(want - 0.75).max() > 0.05   # the mirrored weights do not sum to 1
+    y, x = np.mgrid[0:20, 0:24].astype(np.float32)
+    ramp = (0.25 * x + 0.5 * y).astype(np.float32)
+    nn = np.zeros((40, 48), np.float32)
+    ob.resample(ramp, nn, capi.INTERP_NN, boundary=capi.CLAMP)
+    np.testing.assert_array_equal(nn, np.repeat(np.repeat(ramp, 2, 0), 2, 1))
+
+
+def test_interpolation_l3_reference_rows():
+    """L3 reads rows y_int - 1 .. y_int + 4 weighted lanczos(y_frac - 2 + k),
+    and the second row's fifth tap takes the sixth tap's weight, as the
+    reference (dsl/image.hpp:455-532): an image that is zero except one row
+    shows which source rows feed an output row."""
+    src = np.zeros((16, 16), np.float32)
+    src[8, :] = 1.0
+    out = np.zeros((16, 16), np.float32)
+    ob.resample(src, out, capi.INTERP_L3, boundary=capi.CLAMP)
+    # same size: frac = 0; row y reads rows y - 1 .. y + 4 with weights
+    # l3(-2), l3(-1), l3(0) = 1, l3(1), l3(2), l3(3) = 0 -> only row y+1
+    # contributes (weight 1) among the rows with nonzero weight at integer
+    # distances, so output row 7 holds the source row 8
+    np.testing.assert_allclose(out[7], 1.0, atol=1e-6)
+    assert np.all(np.abs(np.delete(out, 7, axis=0)) < 1e-6)

@@ -132,3 +132,55 @@ def test_pyramid_plan_equals_per_level_launches(gpu):
             assert torch.equal(lp, lap[i][l]), f"image {i} L[{l}]"
             G.append(g1)
     plan.close()
+
+
+# ---- the interpolation set: NN / LF / B5 / CF / L3 ------------------------------
+import golden_cases as gc  # noqa: E402
+
+INTERP_CASES = [(c, e) for c, e in gc.load() if c["op"] in ("interp_f32", "interp_u8")]
+
+
+def _check_interp(got, want, interp, what):
+    if interp == capi.INTERP_L3:
+        # Lanczos weights are evaluated in double with the device's sin; the
+        # weight rounded to float can differ from glibc's in its last bit
+        if got.dtype == np.uint8:
+            d = np.abs(got.astype(np.int32) - want.astype(np.int32))
+            assert d.max() <= 1 and np.count_nonzero(d) <= max(2, got.size // 1000), what
+        else:
+            np.testing.assert_allclose(got, want, rtol=2e-6, atol=2e-6, err_msg=what)
+    else:
+        np.testing.assert_array_equal(got, want, err_msg=what)
+
+
+@pytest.mark.parametrize("case,expected", INTERP_CASES,
+                         ids=[gc.case_id(c) for c, _ in INTERP_CASES])
+def test_interp_golden(gpu, case, expected):
+    """GPU resampling against the reference DSL's own output."""
+    src = gc.input_for(case)
+    interp, shape, kw = gc.interp_spec(case)
+    out = torch.zeros(shape, dtype=torch.uint8 if src.dtype == np.uint8 else torch.float32,
+                      device="cuda")
+    op.resample(to_gpu(src), out, interp, **kw)
+    _check_interp(to_host(out, src.dtype), expected, interp, gc.case_id(case))
+
+
+@pytest.mark.parametrize("interp", [capi.INTERP_NN, capi.INTERP_LF, capi.INTERP_B5,
+                                    capi.INTERP_CF, capi.INTERP_L3])
+@pytest.mark.parametrize("mode", [capi.CLAMP, capi.MIRROR, capi.REPEAT, capi.CONSTANT])
+@pytest.mark.parametrize("dt", [np.float32, np.uint8])
+def test_interp_vs_oracle(gpu, interp, mode, dt):
+    """Down- and up-sampling of larger images, a cropped accessor window and
+    call offsets, against the oracle."""
+    print(f"interp {interp} mode {mode} {np.dtype(dt)}", flush=True)
+    for shape_in, shape_out, win, xf, yf in (((300, 517), (121, 233), None, 0, 0),
+                                             ((97, 130), (333, 517), None, 3, -2),
+                                             ((200, 240), (150, 170), (17, 9, 180, 150), -1, 4)):
+        src = ob.random_image(shape_in, dt, 70 + interp)
+        ref = np.zeros(shape_out, dt)
+        ob.resample(src, ref, interp, boundary=mode, const_val=7, src_window=win, xf=xf, yf=yf)
+        out = torch.zeros(shape_out, dtype=torch.uint8 if dt == np.uint8 else torch.float32,
+                          device="cuda")
+        op.resample(to_gpu(src), out, interp, boundary=mode, const_val=7, src_window=win,
+                    xf=xf, yf=yf)
+        _check_interp(to_host(out, dt), ref, interp, f"{shape_in}->{shape_out} win {win}")

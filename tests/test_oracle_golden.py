@@ -51,5 +51,10 @@ def test_oracle_matches_reference_dsl(case, expected):
         interp = capi.INTERP_NN if op == "nn_f32" else capi.INTERP_LF
         ob.resample(src, dst, interp, boundary=case["boundary"])
         np.testing.assert_array_equal(dst, expected)
+    elif op in ("interp_f32", "interp_u8"):
+        interp, shape, kw = gc.interp_spec(case)
+        dst = np.zeros(shape, src.dtype)
+        ob.resample(src, dst, interp, **kw)
+        np.testing.assert_array_equal(dst, expected)
     else:
         pytest.fail(f"unmapped case {op}")
