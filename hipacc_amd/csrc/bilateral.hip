@@ -69,10 +69,23 @@ __global__ __launch_bounds__(256) void bilateral_kernel(const BilatArgs a) {
   const float k2 = a.k2;
 #pragma unroll 1
   for (int dy = 0; dy < KS; ++dy) {
-    float win[M];
-    const float* row = tile + (tr + dy) * LW + HV * VEC - R + tc;
+    // the row window, read as the 1 + 2 HV aligned 16-byte words that hold
+    // it: a wave's 16-lane row groups read consecutive words (no LDS bank
+    // conflicts; single floats at a 4-word lane stride hit 8 banks only)
+    constexpr int NK = 1 + 2 * HV;
+    float flat[4 * NK];
+    const float4* row4 = reinterpret_cast<const float4*>(tile + (tr + dy) * LW + tc);
 #pragma unroll
-    for (int m = 0; m < M; ++m) win[m] = row[m];
+    for (int k = 0; k < NK; ++k) {
+      const float4 q = row4[k];
+      flat[4 * k] = q.x;
+      flat[4 * k + 1] = q.y;
+      flat[4 * k + 2] = q.z;
+      flat[4 * k + 3] = q.w;
+    }
+    float win[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) win[m] = flat[HV * VEC - R + m];
 #pragma unroll
     for (int dx = 0; dx < KS; ++dx) {
       const float l = a.lsp[dy * KS + dx];
