@@ -235,6 +235,63 @@ def bench_config4(op, capi, dev, steps):
             "modes": out, "mpix_per_step": B * h * w / 1e6}
 
 
+def bench_fusion(op, capi, masks, dev, f32_in, f32_out, steps):
+    """Kernel fusion at 4096^2 float32: the fusion samples' Mixed chain
+    (point, point, 5x5 local, point, point, 5x5 local, point, point; points
+    v*1.0 + 0.25, locals the 5x5 Gaussian, CLAMP) and the P2L chain (point,
+    5x5 local) as one hipacc_hip_fused launch against the same chain as
+    separate launches through intermediate images.  Both are checked equal
+    before timing; bytes per pixel are the fused launch's (read in, write
+    out)."""
+    import torch
+    g5 = masks.gaussian_2d(5, 1.0)
+    pt = dict(kind="point", dtype=capi.F32, op=capi.POINT_AFFINE, alpha=1.0, beta=0.25)
+    lc = dict(kind="local", dtype=capi.F32, mask_w=5, mask_h=5, weights=g5,
+              boundary=capi.CLAMP, fp_contract=True)
+    tmp = [torch.empty_like(f32_in[0]) for _ in range(2)]
+    out = {}
+    for name, chain in (("mixed_pplpplpp_f32", [pt, pt, lc, pt, pt, lc, pt, pt]),
+                        ("p2l_pl_f32", [pt, lc])):
+        fused = [op.prepare_fused(f32_in[i], f32_out[i], chain) for i in range(len(f32_in))]
+
+        def unfused_calls(i, chain=chain):
+            calls, cur = [], f32_in[i]
+            for k, st in enumerate(chain):
+                nxt = f32_out[i] if k == len(chain) - 1 else tmp[k % 2]
+                st = dict(st)
+                kind = st.pop("kind")
+                st.pop("dtype")
+                if kind == "point":
+                    calls.append(op.prepare_point(st.pop("op"), cur, nxt, **st))
+                else:
+                    calls.append(op.prepare_local(cur, nxt, st.pop("mask_w"), st.pop("mask_h"),
+                                                  **st))
+                cur = nxt
+            return calls
+
+        class Seq:
+            def __init__(self, calls):
+                self.calls = calls
+
+            def launch(self):
+                for c in self.calls:
+                    c.launch()
+
+        unfused = [Seq(unfused_calls(i)) for i in range(len(f32_in))]
+        unfused[0].launch()
+        ref = f32_out[0].clone()
+        fused[0].launch()
+        torch.cuda.synchronize()
+        same = bool(torch.equal(ref, f32_out[0]))
+        fms, _ = median_ms(fused, steps, 5, batches=3)
+        ums, _ = median_ms(unfused, max(10, steps // 4), 2, batches=3)
+        e = op_entry(name, fms, 8)
+        e.update({"stages": len(chain), "unfused_ms": round(ums, 5),
+                  "speedup_vs_unfused": round(ums / fms, 3), "fused_equals_unfused": same})
+        out[name] = e
+    return out
+
+
 def pyramid_workload_shapes(n: int = 64, seed: int = 5005):
     """(H, W) of the batched-pyramid workload: W ~ U{256..4096},
     H ~ U{256..2160}, seeded."""
@@ -593,6 +650,7 @@ def main() -> int:
         run("histogram256_u8", [op.HistogramCall(u8_in[i], 256, bins[i]) for i in range(NSETS)], 1,
             tune=False)
         ops["bilateral13_f32_mirror"] = bench_config3(op, capi, dev, 12)
+        result["fusion"] = bench_fusion(op, capi, masks, dev, f32_in, f32_out, steps)
         result["fwd_bwd"] = bench_config4(op, capi, dev, 20)
         result["pyramid"] = bench_config5(op, capi, dev, 20)
         del u8_in, u8_out, i16_out

@@ -15,6 +15,10 @@ LIB_PATH = Path(__file__).resolve().parent / "libhipacc_gfx950.so"
 
 # ---- enums (values as in include/hipacc_hip.h) -----------------------------
 U8, I8, U16, I16, U32, I32, F32 = range(7)
+# 4-wide vector pixels (uchar4, short4, int4, float4): arrays / tensors of
+# shape (..., H, W, 4), channels interleaved
+U8X4, I16X4, I32X4, F32X4 = 7, 8, 9, 10
+VECTOR_DTYPES = (U8X4, I16X4, I32X4, F32X4)
 DTYPE_BYTES = {U8: 1, I8: 1, U16: 2, I16: 2, U32: 4, I32: 4, F32: 4}
 UNDEFINED, CLAMP, REPEAT, MIRROR, CONSTANT = range(5)
 BOUNDARY_NAMES = {UNDEFINED: "UNDEFINED", CLAMP: "CLAMP", REPEAT: "REPEAT", MIRROR: "MIRROR",
@@ -42,6 +46,22 @@ class LocalDesc(ctypes.Structure):
                 ("domain", c_void_p), ("use_weights", c_int32), ("acc_dtype", c_int32),
                 ("reduce", c_int32), ("boundary", c_int32), ("const_val", c_double),
                 ("cvt", c_int32), ("post_div", c_float), ("fp_contract", c_int32)]
+
+
+MAX_FUSED_STAGES = 8
+MAX_FUSED_LOCALS = 2
+STAGE_POINT, STAGE_LOCAL = 0, 1
+
+
+class FusedStage(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("dtype", c_int32), ("op", c_int32), ("alpha", c_float),
+                ("beta", c_float), ("cvt", c_int32), ("in1", View),
+                ("local", POINTER(LocalDesc))]
+
+
+class FusedDesc(ctypes.Structure):
+    _fields_ = [("input", View), ("output", View), ("num_stages", c_int32),
+                ("stages", FusedStage * MAX_FUSED_STAGES)]
 
 
 class SeparableDesc(ctypes.Structure):
@@ -124,6 +144,7 @@ SIGNATURES = {
     "hipacc_hip_local": (c_int32, [POINTER(LocalDesc), c_void_p]),
     "hipacc_hip_separable": (c_int32, [POINTER(SeparableDesc), c_void_p]),
     "hipacc_hip_point": (c_int32, [POINTER(PointDesc), c_void_p]),
+    "hipacc_hip_fused": (c_int32, [POINTER(FusedDesc), c_void_p]),
     "hipacc_hip_reduce_workspace_bytes": (c_size_t, []),
     "hipacc_hip_reduce": (c_int32, [POINTER(View), c_int32, c_int32, c_void_p, c_void_p, c_void_p]),
     "hipacc_hip_reduce_sync": (c_int32, [POINTER(View), c_int32, c_int32, c_void_p, c_void_p]),

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "hh_fused.hpp"
 #include "hh_grad.hpp"
 #include "hh_pyramid.hpp"
 #include "hh_tile.hpp"
@@ -190,7 +191,20 @@ struct Timer {
 // ---------------------------------------------------------------------------
 // Descriptor checks
 // ---------------------------------------------------------------------------
-static bool valid_dtype(int dt) { return dt >= HIPACC_U8 && dt <= HIPACC_F32; }
+static bool valid_dtype(int dt) { return dt >= HIPACC_U8 && dt <= HIPACC_F32X4; }
+
+// The scalar view of a vector-pixel view: its channels as an image of 4x
+// the columns (elementwise operators only).
+static hipacc_view channel_view(const hipacc_view& v) {
+  hipacc_view c = v;
+  c.dtype = dtype_base(v.dtype);
+  c.stride *= 4;
+  c.img_width *= 4;
+  c.width *= 4;
+  c.offset_x *= 4;
+  c.plane_stride *= 4;
+  return c;
+}
 
 static int check_view(const hipacc_view& v, const char* name) {
   if (!v.data) return set_error(HIPACC_ERR_INVALID_ARG, "%s: data is NULL", name);
@@ -360,6 +374,9 @@ static int prep_local(const hipacc_local_desc* d, LocalArgs& a, int& accf, bool&
     return set_error(HIPACC_ERR_INVALID_ARG, "local: acc_dtype must be F32 or I32");
   if (d->reduce < HIPACC_REDUCE_SUM || d->reduce > HIPACC_REDUCE_PROD)
     return set_error(HIPACC_ERR_INVALID_ARG, "local: reduce %d", d->reduce);
+  const bool vecpx = dtype_is_vec(d->in.dtype);
+  if (vecpx != dtype_is_vec(d->out.dtype))
+    return set_error(HIPACC_ERR_INVALID_ARG, "local: vector and scalar pixels mixed");
   const int n = d->mask_w * d->mask_h;
   accf = d->acc_dtype == HIPACC_F32;
   if (d->use_weights) {
@@ -403,7 +420,7 @@ static int prep_local(const hipacc_local_desc* d, LocalArgs& a, int& accf, bool&
   // product of two all-ones rows; integer sums are exact in any order, so
   // the separable running-sum kernels compute the same values.  CONSTANT
   // qualifies only with the constant 0 (otherwise the border rows differ).
-  box = !accf && a.reduce == HIPACC_REDUCE_SUM && !a.wdev && d->in.width == d->out.width &&
+  box = !vecpx && !accf && a.reduce == HIPACC_REDUCE_SUM && !a.wdev && d->in.width == d->out.width &&
         d->in.height == d->out.height && a.mask_w <= 2 * HIPACC_MAX_SEP_RADIUS + 1 &&
         (a.g.boundary != HIPACC_BOUNDARY_CONSTANT || a.g.const_i == 0);
   for (int t = 0; box && t < n; ++t) box = a.domain[t] && (!a.use_weights || a.w.i[t] == 1);
@@ -426,6 +443,9 @@ static int prep_separable(const hipacc_separable_desc* d, SepArgs& a, int& accf)
   std::memset(&a, 0, sizeof(a));
   int st = make_geom(a.g, d->in, d->out, d->boundary, d->const_val, d->cvt, d->post_div);
   if (st) return st;
+  if (dtype_is_vec(d->in.dtype) || dtype_is_vec(d->out.dtype))
+    return set_error(HIPACC_ERR_UNSUPPORTED,
+                     "separable: vector pixels (use hipacc_hip_local with the 2D mask)");
   const int kmax = 2 * HIPACC_MAX_SEP_RADIUS + 1;
   if (d->ksize_x < 1 || d->ksize_y < 1 || d->ksize_x > kmax || d->ksize_y > kmax ||
       !(d->ksize_x & 1) || !(d->ksize_y & 1))
@@ -599,6 +619,155 @@ static int tune_candidates(RunFn run, const TuneKey& key, int iters, hipStream_t
   return HIPACC_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Fused chains
+// ---------------------------------------------------------------------------
+static bool fused_dtype(int dt) {
+  return dt == HIPACC_U8 || dt == HIPACC_I8 || dt == HIPACC_U16 || dt == HIPACC_I16 ||
+         dt == HIPACC_I32 || dt == HIPACC_F32;
+}
+
+static int prep_fused_local(const hipacc_local_desc* l, int in_dt, int out_dt, FusedLocal& L,
+                            int k) {
+  if (!l) return set_error(HIPACC_ERR_INVALID_ARG, "fused: stage %d: local is NULL", k);
+  if (l->mask_w < 1 || l->mask_h < 1 || l->mask_w > HIPACC_MAX_MASK_DIM ||
+      l->mask_h > HIPACC_MAX_MASK_DIM || !(l->mask_w & 1) || !(l->mask_h & 1))
+    return set_error(HIPACC_ERR_INVALID_ARG, "fused: stage %d: mask %dx%d (odd, <= %d)", k,
+                     l->mask_w, l->mask_h, HIPACC_MAX_MASK_DIM);
+  if (l->acc_dtype != HIPACC_F32 && l->acc_dtype != HIPACC_I32)
+    return set_error(HIPACC_ERR_INVALID_ARG, "fused: stage %d: acc_dtype must be F32 or I32", k);
+  if (l->reduce < HIPACC_REDUCE_SUM || l->reduce > HIPACC_REDUCE_PROD)
+    return set_error(HIPACC_ERR_INVALID_ARG, "fused: stage %d: reduce %d", k, l->reduce);
+  if (l->boundary < HIPACC_BOUNDARY_UNDEFINED || l->boundary > HIPACC_BOUNDARY_CONSTANT)
+    return set_error(HIPACC_ERR_INVALID_ARG, "fused: stage %d: boundary %d", k, l->boundary);
+  if (l->cvt != HIPACC_CVT_TRUNC && l->cvt != HIPACC_CVT_ROUND)
+    return set_error(HIPACC_ERR_INVALID_ARG, "fused: stage %d: cvt %d", k, l->cvt);
+  if (l->use_weights && l->weights_dev)
+    return set_error(HIPACC_ERR_UNSUPPORTED, "fused: stage %d: device weights", k);
+  const int n = l->mask_w * l->mask_h;
+  L.acc_f = l->acc_dtype == HIPACC_F32;
+  if (l->use_weights) {
+    if (!l->weights_f32 && !l->weights_i32)
+      return set_error(HIPACC_ERR_INVALID_ARG, "fused: stage %d: use_weights without weights", k);
+    for (int t = 0; t < n; ++t) {
+      if (L.acc_f) L.w.f[t] = l->weights_f32 ? l->weights_f32[t] : (float)l->weights_i32[t];
+      else L.w.i[t] = l->weights_i32 ? l->weights_i32[t] : (int32_t)l->weights_f32[t];
+    }
+  }
+  bool any = false;
+  for (int t = 0; t < n; ++t) {
+    L.domain[t] = l->domain ? (l->domain[t] != 0) : 1;
+    any = any || L.domain[t];
+  }
+  if (!any) return set_error(HIPACC_ERR_INVALID_ARG, "fused: stage %d: empty domain", k);
+  L.mask_w = l->mask_w;
+  L.mask_h = l->mask_h;
+  L.reduce = l->reduce;
+  L.use_weights = l->use_weights != 0;
+  L.fma = l->fp_contract != 0;
+  L.boundary = l->boundary;
+  L.cvt = l->cvt;
+  L.post_div = l->post_div;
+  L.in_dtype = in_dt;
+  L.out_dtype = out_dt;
+  const_views(in_dt, l->const_val, &L.const_f, &L.const_i);
+  return HIPACC_OK;
+}
+
+static int prep_fused(const hipacc_fused_desc* d, FusedArgs& a) {
+  if (!d) return set_error(HIPACC_ERR_INVALID_ARG, "fused: desc is NULL");
+  std::memset(&a, 0, sizeof(a));
+  int st = check_view(d->in, "in");
+  if (st) return st;
+  if ((st = check_view(d->out, "out"))) return st;
+  const hipacc_view& iv = d->in;
+  const hipacc_view& ov = d->out;
+  if (!fused_dtype(iv.dtype) || !fused_dtype(ov.dtype))
+    return set_error(HIPACC_ERR_UNSUPPORTED, "fused: pixel types %d -> %d", iv.dtype, ov.dtype);
+  if (iv.width != ov.width || iv.height != ov.height || batch_of(iv) != batch_of(ov))
+    return set_error(HIPACC_ERR_INVALID_ARG, "fused: in (%dx%d) and out (%dx%d) differ in size",
+                     iv.width, iv.height, ov.width, ov.height);
+  if (d->num_stages < 1 || d->num_stages > HIPACC_MAX_FUSED_STAGES)
+    return set_error(HIPACC_ERR_INVALID_ARG, "fused: %d stages (1..%d)", d->num_stages,
+                     HIPACC_MAX_FUSED_STAGES);
+  int dt = iv.dtype, np = 0, seg = 0;
+  a.seg[0][0] = 0;
+  for (int k = 0; k < d->num_stages; ++k) {
+    const hipacc_fused_stage& g = d->stages[k];
+    if (!fused_dtype(g.dtype))
+      return set_error(HIPACC_ERR_UNSUPPORTED, "fused: stage %d pixel type %d", k, g.dtype);
+    if (g.kind == HIPACC_STAGE_LOCAL) {
+      if (a.num_locals == HIPACC_MAX_FUSED_LOCALS)
+        return set_error(HIPACC_ERR_UNSUPPORTED, "fused: more than %d local stages",
+                         HIPACC_MAX_FUSED_LOCALS);
+      if ((st = prep_fused_local(g.local, dt, g.dtype, a.loc[a.num_locals], k))) return st;
+      ++a.num_locals;
+      a.seg[seg][1] = np;
+      a.seg[++seg][0] = np;
+    } else if (g.kind == HIPACC_STAGE_POINT) {
+      if (g.op < HIPACC_POINT_COPY || g.op > HIPACC_POINT_MAGNITUDE)
+        return set_error(HIPACC_ERR_INVALID_ARG, "fused: stage %d: op %d", k, g.op);
+      if (g.cvt != HIPACC_CVT_TRUNC && g.cvt != HIPACC_CVT_ROUND)
+        return set_error(HIPACC_ERR_INVALID_ARG, "fused: stage %d: cvt %d", k, g.cvt);
+      FusedPoint& p = a.pt[np++];
+      p.op = g.op;
+      p.cvt = g.cvt;
+      p.dtype = g.dtype;
+      p.alpha = g.alpha;
+      p.beta = g.beta;
+      const bool binary = g.op >= HIPACC_POINT_ADD;
+      if (binary) {
+        if ((st = check_view(g.in1, "in1"))) return st;
+        if (g.in1.dtype != dt)
+          return set_error(HIPACC_ERR_UNSUPPORTED,
+                           "fused: stage %d: in1 pixel type %d, the stage's input is %d", k,
+                           g.in1.dtype, dt);
+        if (g.in1.width != ov.width || g.in1.height != ov.height || batch_of(g.in1) != batch_of(ov))
+          return set_error(HIPACC_ERR_INVALID_ARG, "fused: stage %d: in1 window size", k);
+        p.in1.data = g.in1.data;
+        p.in1.plane = g.in1.plane_stride;
+        p.in1.stride = g.in1.stride;
+        p.in1.ox = g.in1.offset_x;
+        p.in1.oy = g.in1.offset_y;
+        p.in1.dtype = g.in1.dtype;
+      }
+    } else {
+      return set_error(HIPACC_ERR_INVALID_ARG, "fused: stage %d: kind %d", k, g.kind);
+    }
+    dt = g.dtype;
+  }
+  a.seg[seg][1] = np;
+  if (a.num_locals == 0) {
+    a.seg[2][0] = a.seg[0][0];
+    a.seg[2][1] = a.seg[0][1];
+  } else if (a.num_locals == 1) {
+    a.seg[2][0] = a.seg[1][0];
+    a.seg[2][1] = a.seg[1][1];
+    a.seg[1][0] = a.seg[1][1] = 0;
+  }
+  if (ov.dtype != dt)
+    return set_error(HIPACC_ERR_INVALID_ARG, "fused: out pixel type %d, the last stage writes %d",
+                     ov.dtype, dt);
+  a.in = iv.data;
+  a.out = ov.data;
+  a.in_plane = iv.plane_stride;
+  a.out_plane = ov.plane_stride;
+  a.in_stride = iv.stride;
+  a.out_stride = ov.stride;
+  a.in_dtype = iv.dtype;
+  a.out_dtype = ov.dtype;
+  a.in_imgw = iv.img_width;
+  a.in_imgh = iv.img_height;
+  a.in_ox = iv.offset_x;
+  a.in_oy = iv.offset_y;
+  a.out_ox = ov.offset_x;
+  a.out_oy = ov.offset_y;
+  a.w = ov.width;
+  a.h = ov.height;
+  a.batch = batch_of(ov);
+  return HIPACC_OK;
+}
+
 }  // namespace hh
 
 extern "C" {
@@ -698,8 +867,21 @@ int hipacc_hip_tune_load(const char* path) {
   return tune_load(path);
 }
 
-int hipacc_hip_point(const hipacc_point_desc* d, void* stream) {
-  if (!d) return set_error(HIPACC_ERR_INVALID_ARG, "point: desc is NULL");
+int hipacc_hip_point(const hipacc_point_desc* d0, void* stream) {
+  if (!d0) return set_error(HIPACC_ERR_INVALID_ARG, "point: desc is NULL");
+  // Vector pixels: every operator here is elementwise, so a 4-channel image
+  // is the scalar image of its interleaved channels (4x the columns).
+  hipacc_point_desc dv = *d0;
+  const bool bin0 = d0->op >= HIPACC_POINT_ADD && d0->op <= HIPACC_POINT_MAGNITUDE;
+  const bool vin = dtype_is_vec(d0->in0.dtype), vout = dtype_is_vec(d0->out.dtype);
+  if (vin || vout || (bin0 && dtype_is_vec(d0->in1.dtype))) {
+    if (!vin || !vout || (bin0 && !dtype_is_vec(d0->in1.dtype)))
+      return set_error(HIPACC_ERR_INVALID_ARG, "point: vector and scalar pixels mixed");
+    dv.in0 = channel_view(d0->in0);
+    dv.out = channel_view(d0->out);
+    if (bin0) dv.in1 = channel_view(d0->in1);
+  }
+  const hipacc_point_desc* d = &dv;
   int st = check_view(d->in0, "in0");
   if (st) return st;
   if ((st = check_view(d->out, "out"))) return st;
@@ -747,6 +929,15 @@ int hipacc_hip_point(const hipacc_point_desc* d, void* stream) {
   return tm.finish(launch_point(a, d->in0.dtype, binary ? d->in1.dtype : d->in0.dtype, o.dtype, s));
 }
 
+int hipacc_hip_fused(const hipacc_fused_desc* d, void* stream) {
+  FusedArgs a;
+  const int st = prep_fused(d, a);
+  if (st) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Timer tm(s);
+  return tm.finish(launch_fused(a, s));
+}
+
 size_t hipacc_hip_reduce_workspace_bytes(void) { return kRedWorkspace; }
 
 int hipacc_hip_reduce(const hipacc_view* img, int32_t reduce, int32_t acc_dtype, void* result_dev,
@@ -755,6 +946,7 @@ int hipacc_hip_reduce(const hipacc_view* img, int32_t reduce, int32_t acc_dtype,
   int st = check_view(*img, "img");
   if (st) return st;
   if (batch_of(*img) != 1) return set_error(HIPACC_ERR_UNSUPPORTED, "reduce: batch > 1");
+  if (dtype_is_vec(img->dtype)) return set_error(HIPACC_ERR_UNSUPPORTED, "reduce: vector pixels");
   if (reduce < HIPACC_REDUCE_SUM || reduce > HIPACC_REDUCE_PROD)
     return set_error(HIPACC_ERR_INVALID_ARG, "reduce: mode %d", reduce);
   if (acc_dtype != HIPACC_F32 && acc_dtype != HIPACC_U32 && acc_dtype != HIPACC_I32)

@@ -26,11 +26,23 @@ template <> struct dtype_of<uint32_t> { static constexpr int value = HIPACC_U32;
 template <> struct dtype_of<int32_t>  { static constexpr int value = HIPACC_I32; };
 template <> struct dtype_of<float>    { static constexpr int value = HIPACC_F32; };
 
-HH_HD int dtype_bytes(int dt) {
+// 4-wide vector pixels: four interleaved channels of the base scalar type.
+HH_HD bool dtype_is_vec(int dt) { return dt >= HIPACC_U8X4 && dt <= HIPACC_F32X4; }
+HH_HD int dtype_base(int dt) {
   switch (dt) {
-    case HIPACC_U8: case HIPACC_I8: return 1;
-    case HIPACC_U16: case HIPACC_I16: return 2;
-    case HIPACC_U32: case HIPACC_I32: case HIPACC_F32: return 4;
+    case HIPACC_U8X4: return HIPACC_U8;
+    case HIPACC_I16X4: return HIPACC_I16;
+    case HIPACC_I32X4: return HIPACC_I32;
+    case HIPACC_F32X4: return HIPACC_F32;
+    default: return dt;
+  }
+}
+HH_HD int dtype_bytes(int dt) {
+  const int lanes = dtype_is_vec(dt) ? 4 : 1;
+  switch (dtype_base(dt)) {
+    case HIPACC_U8: case HIPACC_I8: return 1 * lanes;
+    case HIPACC_U16: case HIPACC_I16: return 2 * lanes;
+    case HIPACC_U32: case HIPACC_I32: case HIPACC_F32: return 4 * lanes;
     default: return 0;
   }
 }

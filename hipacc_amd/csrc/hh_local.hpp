@@ -84,6 +84,8 @@ int tune_load(const char* path);
 // Launch one local operator; returns a HIPACC_* status.  variant/param
 // force a kernel variant (kVarAuto: the tuned or default choice).
 int launch_local(const LocalArgs& a, int acc_is_float, hipStream_t stream, int variant, int param);
+// Local operators on 4-wide vector pixels (local_vec4.hip).
+int launch_local_vec4(const LocalArgs& a, int acc_is_float, hipStream_t s);
 // Fast weighted-SUM instantiations, one table per pixel-type family; each
 // returns HIPACC_ERR_UNSUPPORTED when it has no matching instantiation.
 int dispatch_sum_f32(const LocalArgs& a, int ppt, hipStream_t s);

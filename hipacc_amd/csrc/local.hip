@@ -124,6 +124,9 @@ static int launch_local_variant(const LocalArgs& f, int acc_is_float, hipStream_
 // A forced variant returns HIPACC_ERR_UNSUPPORTED when it has no kernel for
 // the operator's shape.
 int launch_local(const LocalArgs& a, int acc_is_float, hipStream_t s, int variant, int param) {
+  if (dtype_is_vec(a.g.in_dtype))
+    return variant == kVarAuto || variant == kVarGeneric ? launch_local_vec4(a, acc_is_float, s)
+                                                         : HIPACC_ERR_UNSUPPORTED;
   // The tile and stream kernels implement SUM with weights (or a full domain);
   // domain holes become zero weights, which leaves every finite sum unchanged.
   bool fast = a.reduce == HIPACC_REDUCE_SUM && a.mask_w == a.mask_h;

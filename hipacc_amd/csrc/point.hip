@@ -4,6 +4,7 @@
 // iteration space map to blockIdx.y so no per-pixel division is needed.  Loads
 // and stores are issued as one vector when the windows are suitably aligned.
 #include "hh_tile.hpp"
+#include "hh_point.hpp"
 
 namespace hh {
 
@@ -15,38 +16,6 @@ template <typename T> __device__ __forceinline__ void load4(const T* p, bool vec
   } else {
 #pragma unroll
     for (int k = 0; k < 4; ++k) v[k] = k < n ? p[k] : T(0);
-  }
-}
-
-template <typename A> __device__ __forceinline__ A point_eval(int op, A a, A b, float alpha, float beta);
-template <> __device__ __forceinline__ float point_eval<float>(int op, float a, float b, float alpha,
-                                                               float beta) {
-  switch (op) {
-    case HIPACC_POINT_AFFINE: return alpha * a + beta;
-    case HIPACC_POINT_ADD: return a + b;
-    case HIPACC_POINT_SUB: return a - b;
-    case HIPACC_POINT_MUL: return a * b;
-    case HIPACC_POINT_ABSDIFF: return fabsf(a - b);
-    case HIPACC_POINT_MIN: return a < b ? a : b;
-    case HIPACC_POINT_MAX: return a > b ? a : b;
-    case HIPACC_POINT_MAGNITUDE: {
-      float r = sqrtf(a * a + b * b);
-      r = r < 255.0f ? r : 255.0f;
-      return r > 0.0f ? r : 0.0f;
-    }
-    default: return a;
-  }
-}
-template <> __device__ __forceinline__ int32_t point_eval<int32_t>(int op, int32_t a, int32_t b,
-                                                                   float alpha, float beta) {
-  switch (op) {
-    case HIPACC_POINT_ADD: return (int32_t)((uint32_t)a + (uint32_t)b);
-    case HIPACC_POINT_SUB: return (int32_t)((uint32_t)a - (uint32_t)b);
-    case HIPACC_POINT_MUL: return (int32_t)((uint32_t)a * (uint32_t)b);
-    case HIPACC_POINT_ABSDIFF: return a > b ? a - b : b - a;
-    case HIPACC_POINT_MIN: return a < b ? a : b;
-    case HIPACC_POINT_MAX: return a > b ? a : b;
-    default: return a;
   }
 }
 
@@ -69,27 +38,9 @@ __global__ __launch_bounds__(256) void point_kernel(const PointArgs a) {
   }
   TO r[4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if constexpr (FLOAT_MATH) {
-      const float f = point_eval<float>(a.op, (float)v0[k], (float)v1[k], a.alpha, a.beta);
-      r[k] = convert_f<TO>(f, a.cvt, 0.0f);
-    } else {
-      if (a.op == HIPACC_POINT_AFFINE) {
-        r[k] = convert_f<TO>(a.alpha * (float)v0[k] + a.beta, a.cvt, 0.0f);
-      } else if (a.op == HIPACC_POINT_MAGNITUDE) {
-        const int32_t nrm = (int32_t)a.alpha;
-        const int32_t q0 = nrm ? (int32_t)v0[k] / nrm : (int32_t)v0[k];
-        const int32_t q1 = nrm ? (int32_t)v1[k] / nrm : (int32_t)v1[k];
-        float f = sqrtf((float)(q0 * q0 + q1 * q1));
-        f = f < 255.0f ? f : 255.0f;
-        f = f > 0.0f ? f : 0.0f;
-        r[k] = convert_f<TO>(f, a.cvt, 0.0f);
-      } else {
-        r[k] = convert_i<TO>(point_eval<int32_t>(a.op, (int32_t)v0[k], (int32_t)v1[k], 0.f, 0.f),
-                             a.cvt, 0.0f);
-      }
-    }
-  }
+  for (int k = 0; k < 4; ++k)
+    r[k] = point_pixel<TO, FLOAT_MATH>(a.op, (float)v0[k], (float)v1[k], (int32_t)v0[k],
+                                       (int32_t)v1[k], a.alpha, a.beta, a.cvt);
   TO* po = static_cast<TO*>(a.out) + b * a.plane_out + (long long)(a.oyo + y) * a.stride_out + a.oxo + x;
   if (a.vec_ok && n == 4) {
     store_vec<TO, 4>(po, r);

@@ -40,21 +40,27 @@ def dtype_code(t: torch.Tensor) -> int:
 
 
 def view(t: torch.Tensor, window: Window = None, dtype: Optional[int] = None) -> capi.View:
-    """hipacc_view of a (H, W) or (B, H, W) device tensor, rows contiguous."""
+    """hipacc_view of a (H, W) or (B, H, W) device tensor, rows contiguous; with
+    a vector dtype (capi.U8X4, I16X4, I32X4, F32X4) of a (H, W, 4) or
+    (B, H, W, 4) tensor of 4-channel pixels."""
     if not t.is_cuda:
         raise ValueError("image tensors must live on the GPU")
-    if t.dim() not in (2, 3) or t.stride(-1) != 1:
-        raise ValueError("images are (H, W) or (B, H, W) tensors with unit column stride")
-    h, w = t.shape[-2], t.shape[-1]
+    vec = dtype in capi.VECTOR_DTYPES
+    nd = t.dim() - (1 if vec else 0)
+    if nd not in (2, 3) or t.stride(-1) != 1 or (vec and (t.shape[-1] != 4 or t.stride(-2) != 4)):
+        raise ValueError("images are (H, W) or (B, H, W) tensors with unit column stride "
+                         "((..., 4) with interleaved channels for vector pixels)")
+    lanes = 4 if vec else 1
+    h, w = (t.shape[-3], t.shape[-2]) if vec else (t.shape[-2], t.shape[-1])
     v = capi.View()
     v.data = t.data_ptr()
     v.dtype = dtype_code(t) if dtype is None else dtype
-    v.stride = t.stride(-2)
+    v.stride = t.stride(-3 if vec else -2) // lanes
     v.img_width, v.img_height = w, h
     x, y, ww, hh = window if window is not None else (0, 0, w, h)
     v.offset_x, v.offset_y, v.width, v.height = x, y, ww, hh
-    v.batch = t.shape[0] if t.dim() == 3 else 1
-    v.plane_stride = t.stride(0) if t.dim() == 3 else 0
+    v.batch = t.shape[0] if nd == 3 else 1
+    v.plane_stride = t.stride(0) // lanes if nd == 3 else 0
     return v
 
 
@@ -118,9 +124,9 @@ def prepare_local(src: torch.Tensor, dst: torch.Tensor, mask_w: int, mask_h: int
           reduce: int = capi.SUM, boundary: int = capi.CLAMP, const_val: float = 0.0,
           cvt: int = capi.CVT_TRUNC, post_div: float = 0.0, fp_contract: bool = True,
           src_window: Window = None, dst_window: Window = None,
-          dst_dtype: Optional[int] = None) -> Call:
+          dst_dtype: Optional[int] = None, src_dtype: Optional[int] = None) -> Call:
     d = capi.LocalDesc()
-    d.input = view(src, src_window)
+    d.input = view(src, src_window, src_dtype)
     d.output = view(dst, dst_window, dst_dtype)
     d.mask_w, d.mask_h = mask_w, mask_h
     keep = []
@@ -153,9 +159,9 @@ def prepare_separable(src: torch.Tensor, dst: torch.Tensor, row: Sequence, col: 
               acc_float: bool = True, boundary: int = capi.CLAMP, const_val: float = 0.0,
               cvt: int = capi.CVT_TRUNC, post_div: float = 0.0, fp_contract: bool = True,
               src_window: Window = None, dst_window: Window = None,
-              dst_dtype: Optional[int] = None) -> Call:
+              dst_dtype: Optional[int] = None, src_dtype: Optional[int] = None) -> Call:
     d = capi.SeparableDesc()
-    d.input = view(src, src_window)
+    d.input = view(src, src_window, src_dtype)
     d.output = view(dst, dst_window, dst_dtype)
     d.ksize_x, d.ksize_y = len(row), len(col)
     if acc_float:
@@ -180,14 +186,62 @@ def prepare_point(op: int, src0: torch.Tensor, dst: torch.Tensor,
                   src1: Optional[torch.Tensor] = None, *,
           alpha: float = 1.0, beta: float = 0.0, cvt: int = capi.CVT_TRUNC,
           windows: Tuple[Window, Window, Window] = (None, None, None),
-          dst_dtype: Optional[int] = None) -> Call:
+          dst_dtype: Optional[int] = None, src_dtype: Optional[int] = None) -> Call:
     d = capi.PointDesc()
-    d.in0 = view(src0, windows[0])
+    d.in0 = view(src0, windows[0], src_dtype)
     if src1 is not None:
-        d.in1 = view(src1, windows[1])
+        d.in1 = view(src1, windows[1], src_dtype)
     d.output = view(dst, windows[2], dst_dtype)
     d.op, d.alpha, d.beta, d.cvt = op, alpha, beta, cvt
     return Call("hipacc_hip_point", d)
+
+
+def fused(*args, **kw) -> None:
+    prepare_fused(*args, **kw).launch()
+
+
+def prepare_fused(src: torch.Tensor, dst: torch.Tensor, stages, *, src_window: Window = None,
+                  dst_window: Window = None, src_dtype: Optional[int] = None,
+                  dst_dtype: Optional[int] = None) -> Call:
+    """hipacc_hip_fused: a chain of point and local stages in one launch.
+
+    Each stage is a dict with ``kind`` ("point" or "local") and ``dtype``
+    (capi pixel type of the image it writes).  A point stage takes ``op``
+    (default POINT_AFFINE), ``alpha``, ``beta``, ``cvt`` and, for two-input
+    ops, ``in1`` (device tensor) with ``in1_window``; a local stage takes
+    prepare_local's keyword arguments (mask_w, mask_h, weights, domain,
+    acc_float, reduce, boundary, const_val, cvt, post_div, fp_contract)."""
+    if not 1 <= len(stages) <= capi.MAX_FUSED_STAGES:
+        raise ValueError(f"1..{capi.MAX_FUSED_STAGES} stages")
+    d = capi.FusedDesc()
+    d.input = view(src, src_window, src_dtype)
+    d.output = view(dst, dst_window, dst_dtype)
+    d.num_stages = len(stages)
+    keep = []
+    for k, st in enumerate(stages):
+        st = dict(st)
+        g = d.stages[k]
+        kind = st.pop("kind")
+        g.dtype = st.pop("dtype")
+        if kind == "point":
+            g.kind = capi.STAGE_POINT
+            g.op = st.pop("op", capi.POINT_AFFINE)
+            g.alpha, g.beta = st.pop("alpha", 1.0), st.pop("beta", 0.0)
+            g.cvt = st.pop("cvt", capi.CVT_TRUNC)
+            in1 = st.pop("in1", None)
+            win1 = st.pop("in1_window", None)
+            if in1 is not None:
+                g.in1 = view(in1, win1, st.pop("in1_dtype", None))
+            if st:
+                raise ValueError(f"point stage {k}: unknown keys {sorted(st)}")
+        elif kind == "local":
+            g.kind = capi.STAGE_LOCAL
+            lc = prepare_local(src, dst, st.pop("mask_w"), st.pop("mask_h"), **st)
+            keep.append(lc)
+            g.local = ctypes.pointer(lc.desc)
+        else:
+            raise ValueError(f"stage {k}: kind {kind!r}")
+    return Call("hipacc_hip_fused", d, keep)
 
 
 class ReduceCall:

@@ -52,10 +52,15 @@ enum {
 
 /* ---- pixel types ------------------------------------------------------- */
 /* The scalar pixel types of the DSL (dsl/types.hpp:33-36 uchar/ushort/uint
- * plus char/short/int/float).                                              */
+ * plus char/short/int/float), and its 4-wide vector pixels (dsl/types.hpp:
+ * 38-100: uchar4, short4, int4, float4; four interleaved channels per
+ * pixel, `stride` and offsets counted in pixels).  A vector operator applies
+ * the scalar operator to each channel: the DSL's vector arithmetic is
+ * componentwise.                                                           */
 typedef enum {
   HIPACC_U8 = 0, HIPACC_I8 = 1, HIPACC_U16 = 2, HIPACC_I16 = 3,
-  HIPACC_U32 = 4, HIPACC_I32 = 5, HIPACC_F32 = 6
+  HIPACC_U32 = 4, HIPACC_I32 = 5, HIPACC_F32 = 6,
+  HIPACC_U8X4 = 7, HIPACC_I16X4 = 8, HIPACC_I32X4 = 9, HIPACC_F32X4 = 10
 } hipacc_dtype;
 
 /* ---- boundary modes: same values as hipacc::Boundary (dsl/image.hpp:46-52) */
@@ -123,6 +128,11 @@ typedef struct hipacc_view {
  * The input coordinate of output pixel (x, y) is
  * (x - out.offset_x + in.offset_x, y - out.offset_y + in.offset_y)
  * (dsl/image.hpp:390-412, Interpolate::NO).
+ * Vector pixels (in and out both vector typed; host weights): each channel
+ * is this operator on that channel, with the constant of CONSTANT in every
+ * channel -- the RGBA samples (samples-public/1_Local_Operators/
+ * Gaussian_Blur_RGBA, Box_Blur_RGBA, Erode_RGBA, Dilate_RGBA, Laplace_RGBA,
+ * 3_Preprocessing/Sobel_RGBA) in the DSL's componentwise arithmetic.
  * acc_dtype is HIPACC_F32 (float lambda, e.g. Gaussian) or HIPACC_I32
  * (int lambda, e.g. Sobel / box sum).  fp_contract = 1 evaluates a float
  * weighted SUM as acc = fma(w, x, acc), the contraction the reference CUDA
@@ -151,6 +161,50 @@ typedef struct hipacc_local_desc {
   int32_t fp_contract;       /* see above                                   */
 } hipacc_local_desc;
 int hipacc_hip_local(const hipacc_local_desc* desc, void* stream);
+
+/* ---- fused operator chains ------------------------------------------- */
+/* Kernel fusion (the reference's compiler pass lib/AST/ASTFuse.cpp:212-413
+ * with its point-to-point, point-to-local, local-to-point and
+ * local-to-local transformations, chains found by lib/Analysis/
+ * HostDataDeps.cpp:903-1374; samples-public/6_Test/Kernel_Fusion_{P2P,P2L,
+ * L2P,L2L,Mixed}): a chain of same-size kernels, each reading the image
+ * the previous one wrote, runs as one launch without the intermediate
+ * images, and its result equals the unfused chain's bit for bit.
+ *
+ * stages[0] reads `in`; stages[k] reads the image stages[k-1] writes;
+ * stages[num_stages-1] writes `out`.  All windows (in, out, in1) have the
+ * size of `out`; the intermediate images are out-sized images at offset 0.
+ *   POINT  hipacc_hip_point's arithmetic: op, alpha, beta, cvt on the
+ *          previous image and, for a two-input op, on `in1` (same pixel
+ *          type as the previous image), read at the stage's own pixel.
+ *   LOCAL  a local operator (`local`: mask, weights, domain, reduce,
+ *          acc_dtype, boundary, const_val, cvt, post_div, fp_contract;
+ *          its `in` and `out` views are ignored) over the previous image
+ *          with that image's boundary handling: a local first stage reads
+ *          `in` through its window as hipacc_hip_local does; a later one
+ *          reads an intermediate image (UNDEFINED is clamped to it).
+ *          At most two LOCAL stages, as in the reference (host weights).
+ * Pixel types of every image: U8, I8, U16, I16, I32, F32; out.dtype must
+ * be the last stage's dtype; `out` must not overlap `in` or any `in1`.   */
+#define HIPACC_MAX_FUSED_STAGES 8
+#define HIPACC_MAX_FUSED_LOCALS 2
+typedef enum hipacc_stage_kind { HIPACC_STAGE_POINT = 0, HIPACC_STAGE_LOCAL = 1 } hipacc_stage_kind;
+typedef struct hipacc_fused_stage {
+  int32_t kind;              /* hipacc_stage_kind                             */
+  int32_t dtype;             /* pixel type of the image the stage writes      */
+  int32_t op;                /* POINT: hipacc_point_op                        */
+  float   alpha, beta;       /* POINT                                         */
+  int32_t cvt;               /* POINT: hipacc_cvt                             */
+  hipacc_view in1;           /* POINT: second input of a two-input op         */
+  const hipacc_local_desc* local;   /* LOCAL                                  */
+} hipacc_fused_stage;
+typedef struct hipacc_fused_desc {
+  hipacc_view in;            /* the first stage's accessor                    */
+  hipacc_view out;           /* the last stage's iteration space              */
+  int32_t num_stages;        /* 1..HIPACC_MAX_FUSED_STAGES                    */
+  hipacc_fused_stage stages[HIPACC_MAX_FUSED_STAGES];
+} hipacc_fused_desc;
+int hipacc_hip_fused(const hipacc_fused_desc* desc, void* stream);
 
 /* ---- separable local operator: a row pass then a column pass ---------- */
 /* The two-kernel DSL program
@@ -194,7 +248,10 @@ int hipacc_hip_separable(const hipacc_separable_desc* desc, void* stream);
  *            Sobel combine kernel (samples-public/3_Preprocessing/Sobel/
  *            src/main.cpp:87-96).
  * Arithmetic is int32 when both inputs are integer types, float otherwise,
- * matching C++'s usual arithmetic conversions for these types.            */
+ * matching C++'s usual arithmetic conversions for these types.
+ * Vector pixels (all views vector typed): each channel, as the scalar
+ * operator (the Sobel_RGBA combine, samples-public/3_Preprocessing/
+ * Sobel_RGBA).                                                              */
 typedef enum {
   HIPACC_POINT_COPY = 0, HIPACC_POINT_AFFINE = 1, HIPACC_POINT_ADD = 2,
   HIPACC_POINT_SUB = 3, HIPACC_POINT_MUL = 4, HIPACC_POINT_ABSDIFF = 5,

@@ -37,11 +37,23 @@
 #define ORC_EXPORT __attribute__((visibility("default")))
 
 /* ---- pixel access ------------------------------------------------------ */
-static int dt_bytes(int dt) {
+/* Vector pixels (uchar4, short4, int4, float4) are four interleaved
+ * channels of their scalar type; channel c of pixel i is element 4*i + c. */
+static int dt_base(int dt) {
   switch (dt) {
-    case HIPACC_U8: case HIPACC_I8: return 1;
-    case HIPACC_U16: case HIPACC_I16: return 2;
-    default: return 4;
+    case HIPACC_U8X4: return HIPACC_U8;
+    case HIPACC_I16X4: return HIPACC_I16;
+    case HIPACC_I32X4: return HIPACC_I32;
+    case HIPACC_F32X4: return HIPACC_F32;
+    default: return dt;
+  }
+}
+static int dt_lanes(int dt) { return dt >= HIPACC_U8X4 ? 4 : 1; }
+static int dt_bytes(int dt) {
+  switch (dt_base(dt)) {
+    case HIPACC_U8: case HIPACC_I8: return 1 * dt_lanes(dt);
+    case HIPACC_U16: case HIPACC_I16: return 2 * dt_lanes(dt);
+    default: return 4 * dt_lanes(dt);
   }
 }
 static double px_get(const void* p, int dt, long long i) {
@@ -189,9 +201,14 @@ ORC_EXPORT int oracle_local(const hipacc_local_desc* d) {
   const hipacc_view* out = &d->out;
   const int mw = d->mask_w, mh = d->mask_h, rx = mw / 2, ry = mh / 2;
   const int accf = d->acc_dtype == HIPACC_F32;
-  const float cf = const_f(in->dtype, d->const_val);
-  const int32_t ci = const_i(in->dtype, d->const_val);
-  for (int b = 0; b < nbatch(out); ++b) {
+  /* vector pixels: the scalar operator on each channel */
+  const int nch = dt_lanes(in->dtype);
+  if (dt_lanes(out->dtype) != nch) return 1;
+  const int ib = dt_base(in->dtype), ob = dt_base(out->dtype);
+  const float cf = const_f(ib, d->const_val);
+  const int32_t ci = const_i(ib, d->const_val);
+  for (int b = 0; b < nbatch(out); ++b)
+  for (int ch = 0; ch < nch; ++ch) {
     const char* ip = plane(in, b);
     char* op = (char*)plane(out, b);
 #pragma omp parallel for schedule(static)
@@ -208,7 +225,7 @@ ORC_EXPORT int oracle_local(const hipacc_local_desc* d) {
             long long idx = 0;
             const int hit = bh_tap(in, d->boundary, cx + dx - rx, cy + dy - ry, &idx);
             if (accf) {
-              const float v = hit ? px_f(ip, in->dtype, idx) : cf;
+              const float v = hit ? px_f(ip, ib, idx * nch + ch) : cf;
               float w = 1.0f;
               if (d->use_weights) {
                 if (d->weights_f32) w = d->weights_f32[t];
@@ -229,7 +246,7 @@ ORC_EXPORT int oracle_local(const hipacc_local_desc* d) {
                 }
               }
             } else {
-              const int32_t v = hit ? px_i(ip, in->dtype, idx) : ci;
+              const int32_t v = hit ? px_i(ip, ib, idx * nch + ch) : ci;
               int32_t w = 1;
               if (d->use_weights) {
                 if (d->weights_i32) w = d->weights_i32[t];
@@ -248,8 +265,8 @@ ORC_EXPORT int oracle_local(const hipacc_local_desc* d) {
           }
         }
         const long long o = (long long)(y + out->offset_y) * out->stride + x + out->offset_x;
-        if (accf) px_put_f(op, out->dtype, o, af, d->cvt, d->post_div);
-        else px_put_i(op, out->dtype, o, ai, d->cvt, d->post_div);
+        if (accf) px_put_f(op, ob, o * nch + ch, af, d->cvt, d->post_div);
+        else px_put_i(op, ob, o * nch + ch, ai, d->cvt, d->post_div);
       }
     }
   }

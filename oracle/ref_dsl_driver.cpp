@@ -24,11 +24,25 @@
 //     nn_f32    <dst_w> <dst_h>           point over a dst_w x dst_h space reading
 //                                         the source through Interpolate::NN
 //     lf_f32    <dst_w> <dst_h>           same through Interpolate::LF
+//     conv_rgba <k> <sigma>  uchar4 Gaussian (the Gaussian_Blur_RGBA kernel):
+//                            convert_uchar4(convolve(SUM, mask()*convert_float4(in(mask))) + 0.5f)
+//     conv_f32x4 <k> <sigma> float4 Gaussian: convolve(SUM, mask()*in(mask))
+//     box_rgba  <k>          uchar4 box (Box_Blur_RGBA): convert_uchar4(
+//                            convert_float4(reduce(dom, SUM, convert_int4(in(dom)))) / (k*k))
+//     erode_rgba <k>         uchar4 erode (Erode_RGBA): reduce(dom, MIN, in(dom))
+//     sobel_rgba <axis>      int4 Sobel (Sobel_RGBA): reduce(dom, SUM, mask(dom)*convert_int4(in(dom)))
 //     interp_f32 / interp_u8 <imode 1..5> <dst_w> <dst_h> <xf> <yf> <cx> <cy> <cw> <ch>
 //                                         output() = in(xf, yf) through Interpolate
 //                                         NN/LF/B5/CF/L3, the accessor a crop
 //                                         (cx, cy, cw, ch; cw = 0: the whole
 //                                         image) with the boundary condition
+//     fuse_u8 / fuse_f32 <stages>  the kernel chains of the fusion samples
+//                            (samples-public/6_Test/Kernel_Fusion_*): each
+//                            letter of <stages> one kernel over the previous
+//                            one's image, P = point `v = in(); v += 3;
+//                            output() = v`, L = local (T)(convolve(5x5
+//                            sample mask, SUM, mask()*in(mask)) + 0.5f)
+//                            with the boundary condition
 #include "hipacc.hpp"
 
 #include <cmath>
@@ -181,6 +195,117 @@ class Resample : public Kernel<float> {
   void kernel() { output() = in(); }
 };
 
+// 4-wide vector pixels: the RGBA samples' kernels
+class ConvRGBA : public Kernel<uchar4> {
+  Accessor<uchar4>& in; Mask<float>& mask;
+ public:
+  ConvRGBA(IterationSpace<uchar4>& is, Accessor<uchar4>& in, Mask<float>& m) : Kernel(is), in(in), mask(m) { add_accessor(&in); }
+  void kernel() {
+    float4 sum = convolve(mask, Reduce::SUM, [&]() -> float4 { return mask() * convert_float4(in(mask)); });
+    output() = convert_uchar4(sum + 0.5f);
+  }
+};
+class ConvF32x4 : public Kernel<float4> {
+  Accessor<float4>& in; Mask<float>& mask;
+ public:
+  ConvF32x4(IterationSpace<float4>& is, Accessor<float4>& in, Mask<float>& m) : Kernel(is), in(in), mask(m) { add_accessor(&in); }
+  void kernel() { output() = convolve(mask, Reduce::SUM, [&]() -> float4 { return mask() * in(mask); }); }
+};
+class BoxRGBA : public Kernel<uchar4> {
+  Accessor<uchar4>& in; Domain& dom; int n;
+ public:
+  BoxRGBA(IterationSpace<uchar4>& is, Accessor<uchar4>& in, Domain& d, int n) : Kernel(is), in(in), dom(d), n(n) { add_accessor(&in); }
+  void kernel() {
+    int4 sum = reduce(dom, Reduce::SUM, [&]() -> int4 { return convert_int4(in(dom)); });
+    output() = convert_uchar4(convert_float4(sum) / (float)n);
+  }
+};
+class ErodeRGBA : public Kernel<uchar4> {
+  Accessor<uchar4>& in; Domain& dom;
+ public:
+  ErodeRGBA(IterationSpace<uchar4>& is, Accessor<uchar4>& in, Domain& d) : Kernel(is), in(in), dom(d) { add_accessor(&in); }
+  void kernel() { output() = reduce(dom, Reduce::MIN, [&]() -> uchar4 { return in(dom); }); }
+};
+class SobelRGBA : public Kernel<int4> {
+  Accessor<uchar4>& in; Domain& dom; Mask<int>& mask;
+ public:
+  SobelRGBA(IterationSpace<int4>& is, Accessor<uchar4>& in, Domain& d, Mask<int>& m) : Kernel(is), in(in), dom(d), mask(m) { add_accessor(&in); }
+  void kernel() { output() = reduce(dom, Reduce::SUM, [&]() -> int4 { return mask(dom) * convert_int4(in(dom)); }); }
+};
+
+static std::vector<uchar4> pixels_rgba(int w, int h, uint64_t seed) {
+  std::vector<uchar4> v((size_t)w * h);
+  for (size_t i = 0; i < v.size(); ++i) {
+    v[i].x = sm64_u8(seed, 4 * i); v[i].y = sm64_u8(seed, 4 * i + 1);
+    v[i].z = sm64_u8(seed, 4 * i + 2); v[i].w = sm64_u8(seed, 4 * i + 3);
+  }
+  return v;
+}
+static std::vector<float4> pixels_f32x4(int w, int h, uint64_t seed) {
+  std::vector<float4> v((size_t)w * h);
+  for (size_t i = 0; i < v.size(); ++i) {
+    v[i].x = sm64_f32(seed, 4 * i); v[i].y = sm64_f32(seed, 4 * i + 1);
+    v[i].z = sm64_f32(seed, 4 * i + 2); v[i].w = sm64_f32(seed, 4 * i + 3);
+  }
+  return v;
+}
+template <int K> static int run_rgba(const std::string& op, int w, int h, uint64_t seed, double sigma,
+                                     const char* path) {
+  float coef[K][K];
+  gauss<K>(coef, sigma);
+  Mask<float> fmask(coef);
+  if (op == "conv_f32x4") {
+    auto px = pixels_f32x4(w, h, seed);
+    Image<float4> in(w, h, px.data()), out(w, h);
+    auto bc = make_bc(in, fmask);
+    Accessor<float4> acc(bc);
+    IterationSpace<float4> is(out);
+    ConvF32x4 k(is, acc, fmask);
+    k.execute();
+    write_out(path, out.data(), (size_t)w * h * 16);
+    return 0;
+  }
+  auto px = pixels_rgba(w, h, seed);
+  Image<uchar4> in(w, h, px.data()), out(w, h);
+  if (op == "conv_rgba") {
+    auto bc = make_bc(in, fmask);
+    Accessor<uchar4> acc(bc);
+    IterationSpace<uchar4> is(out);
+    ConvRGBA k(is, acc, fmask);
+    k.execute();
+  } else {
+    Domain dom(K, K);
+    auto bc = make_bc(in, dom);
+    Accessor<uchar4> acc(bc);
+    IterationSpace<uchar4> is(out);
+    if (op == "box_rgba") {
+      BoxRGBA k(is, acc, dom, K * K);
+      k.execute();
+    } else {
+      ErodeRGBA k(is, acc, dom);
+      k.execute();
+    }
+  }
+  write_out(path, out.data(), (size_t)w * h * 4);
+  return 0;
+}
+static int run_sobel_rgba(int w, int h, uint64_t seed, int axis, const char* path) {
+  const int cx[3][3] = {{-1, 0, 1}, {-2, 0, 2}, {-1, 0, 1}};
+  const int cy[3][3] = {{-1, -2, -1}, {0, 0, 0}, {1, 2, 1}};
+  auto px = pixels_rgba(w, h, seed);
+  Image<uchar4> in(w, h, px.data());
+  Image<int4> out(w, h);
+  Mask<int> mask(axis ? cy : cx);
+  Domain dom(mask);
+  auto bc = make_bc(in, dom);
+  Accessor<uchar4> acc(bc);
+  IterationSpace<int4> is(out);
+  SobelRGBA k(is, acc, dom, mask);
+  k.execute();
+  write_out(path, out.data(), (size_t)w * h * 16);
+  return 0;
+}
+
 template <typename T> class ResampleAt : public Kernel<T> {
   Accessor<T>& in;
   const int xf, yf;
@@ -205,6 +330,64 @@ static int run_interp(int w, int h, uint64_t seed, int imode, int dw, int dh, in
   ResampleAt<T> k(is, acc, xf, yf);
   k.execute();
   write_out(path, out.data(), (size_t)dw * dh * sizeof(T));
+  return 0;
+}
+
+// ---- fusion chains -----------------------------------------------------------
+// The point and local kernels of samples-public/6_Test/Kernel_Fusion_P2L/src/
+// main.cpp:43-78 (the same in the P2P, L2P, L2L and Mixed samples), and
+// their 5x5 mask (main.cpp:91-97).
+template <typename T> class AddThree : public Kernel<T> {
+  Accessor<T>& in;
+ public:
+  AddThree(IterationSpace<T>& is, Accessor<T>& in) : Kernel<T>(is), in(in) { this->add_accessor(&in); }
+  void kernel() {
+    T v = in();
+    v += 3;
+    this->output() = v;
+  }
+};
+template <typename T> class GaussRound : public Kernel<T> {
+  Accessor<T>& in;
+  Mask<float>& mask;
+ public:
+  GaussRound(IterationSpace<T>& is, Accessor<T>& in, Mask<float>& m)
+      : Kernel<T>(is), in(in), mask(m) { this->add_accessor(&in); }
+  void kernel() {
+    this->output() = (T)(this->convolve(mask, Reduce::SUM, [&]() -> float { return mask() * in(mask); }) + 0.5f);
+  }
+};
+static const float kFuseMask[5][5] = {
+    {0.005008f, 0.017300f, 0.026151f, 0.017300f, 0.005008f},
+    {0.017300f, 0.059761f, 0.090339f, 0.059761f, 0.017300f},
+    {0.026151f, 0.090339f, 0.136565f, 0.090339f, 0.026151f},
+    {0.017300f, 0.059761f, 0.090339f, 0.059761f, 0.017300f},
+    {0.005008f, 0.017300f, 0.026151f, 0.017300f, 0.005008f}};
+
+template <typename T>
+static int run_fuse(int w, int h, uint64_t seed, const std::string& stages, const char* path) {
+  std::vector<T> cur = pixels<T>(w, h, seed);
+  Mask<float> mask(kFuseMask);
+  for (char c : stages) {
+    Image<T> in(w, h, cur.data()), out(w, h);
+    IterationSpace<T> is(out);
+    if (c == 'P') {
+      Accessor<T> acc(in);
+      AddThree<T> k(is, acc);
+      k.execute();
+    } else if (c == 'L') {
+      auto bc = make_bc(in, mask);
+      Accessor<T> acc(bc);
+      GaussRound<T> k(is, acc, mask);
+      k.execute();
+    } else {
+      std::fprintf(stderr, "fuse: stage %c\n", c);
+      return 2;
+    }
+    const T* o = out.data();
+    cur.assign(o, o + (size_t)w * h);
+  }
+  write_out(path, cur.data(), (size_t)w * h * sizeof(T));
   return 0;
 }
 
@@ -412,6 +595,16 @@ int main(int argc, char** argv) {
     write_out(path, out.data(), (size_t)dw * dh * 4);
     return 0;
   }
+  if (op == "conv_rgba" || op == "conv_f32x4" || op == "box_rgba" || op == "erode_rgba") {
+    const int k = std::atoi(arg(0));
+    const double s = std::atof(arg(1));
+    switch (k) {
+      case 3: return run_rgba<3>(op, w, h, seed, s, path);
+      case 5: return run_rgba<5>(op, w, h, seed, s, path);
+      case 7: return run_rgba<7>(op, w, h, seed, s, path);
+    }
+  }
+  if (op == "sobel_rgba") return run_sobel_rgba(w, h, seed, std::atoi(arg(0)), path);
   if (op == "interp_f32" || op == "interp_u8") {
     int v[9];
     for (int i = 0; i < 9; ++i) v[i] = std::atoi(arg(i));
@@ -419,6 +612,8 @@ int main(int argc, char** argv) {
       return run_interp<float>(w, h, seed, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], path);
     return run_interp<uchar>(w, h, seed, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], path);
   }
+  if (op == "fuse_u8") return run_fuse<uchar>(w, h, seed, arg(0), path);
+  if (op == "fuse_f32") return run_fuse<float>(w, h, seed, arg(0), path);
   std::fprintf(stderr, "unknown op/size: %s\n", op.c_str());
   return 2;
 }

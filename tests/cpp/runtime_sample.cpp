@@ -121,6 +121,15 @@ int main(int argc, char** argv) {
   unsigned int* hist = hipaccApplyBinning(hipaccMakeAccessor(in8), 256);
   save("hist_u32", hist, 256);
   delete[] hist;
+  // the same through the names generated host code calls
+  float fmax_shared = hipaccApplyReductionShared<float>(
+      HipaccReduce::MAX, hipaccMakeAccessor(inf, 50, 40, 20, 30), 256, 16, ep);
+  unsigned int* hist2 = hipaccApplyBinningSegmented<unsigned int, unsigned char>(
+      HipaccCountBins{}, hipaccMakeAccessor(in8), 4, 32, 256, ep);
+  save("hist2_u32", hist2, 256);
+  delete[] hist2;
+  // repeated binning reuses the thread's device bins
+  for (int i = 0; i < 3; ++i) delete[] hipaccApplyBinning(hipaccMakeAccessor(in8), 64);
 
   // Pyramid: depth 4; at each level below the top, the level image is the
   // top-left window of the level above (a copy), and a traversal counts the
@@ -143,9 +152,10 @@ int main(int argc, char** argv) {
   std::FILE* f = std::fopen((g_out + "/scalars.txt").c_str(), "w");
   if (!f) return 1;
   std::fprintf(f, "status %d\nfsum %.9g\nfmin %.9g\nfmax %.9g\nusum %u\nvisited %d\n"
-               "pyr3 %d %d\nstride_gauss %d\nstride_sobel %d\ntiming %.9g %.9g\n",
+               "pyr3 %d %d\nstride_gauss %d\nstride_sobel %d\ntiming %.9g %.9g\n"
+               "fmax_shared %.9g\n",
                st, fsum, fmin, fmax, usum, visited, last->get_width(), last->get_height(),
-               gauss->get_stride(), sob->get_stride(), t_on, t_off);
+               gauss->get_stride(), sob->get_stride(), t_on, t_off, fmax_shared);
   std::fclose(f);
   return st == 0 ? 0 : 1;
 }

@@ -85,10 +85,27 @@ def cases():
         (29, 35, 45, 17, 2, 1, (0, 0, 0, 0), "REPEAT"),
         (33, 21, 50, 40, -1, 1, (0, 0, 0, 0), "CONSTANT"),
     )
+    # 4-wide vector pixels: the RGBA samples' operators
+    for (w, h) in SIZES[:2]:
+        for m in MODES:
+            add("conv_rgba", w, h, m, [5, 1.0], "uint8", [h, w, 4])
+            add("conv_rgba", w, h, m, [3, 0.8], "uint8", [h, w, 4])
+            add("conv_f32x4", w, h, m, [5, 1.0], "float32", [h, w, 4])
+            add("box_rgba", w, h, m, [5, 0], "uint8", [h, w, 4])
+            add("erode_rgba", w, h, m, [3, 0], "uint8", [h, w, 4])
+            add("sobel_rgba", w, h, m, [1], "int32", [h, w, 4])
     for imode in (1, 2, 3, 4, 5):
         for dt, op in (("float32", "interp_f32"), ("uint8", "interp_u8")):
             for (w, h, dw, dh, xf, yf, crop, m) in geoms:
                 add(op, w, h, m, [imode, dw, dh, xf, yf, *crop], dt, [dh, dw])
+    # the kernel chains of the fusion samples (P2P, P2L, L2P, L2L, Mixed):
+    # the unfused chain's output, which a fused launch must equal
+    for (w, h) in SIZES[:2]:
+        for m in MODES:
+            for st in ("PPP", "PPL", "LPP", "LL", "PPLPPLPP", "PLPL"):
+                if m == "CLAMP" or "L" in st:
+                    add("fuse_u8", w, h, m, [st], "uint8")
+            add("fuse_f32", w, h, m, ["PLPLP"], "float32")
     return out
 
 

@@ -58,17 +58,24 @@ Window = Optional[Tuple[int, int, int, int]]
 
 
 def hview(a: np.ndarray, window: Window = None, dtype: Optional[int] = None) -> capi.View:
+    """View of a (H, W) / (B, H, W) image, or with a vector dtype (capi.U8X4
+    ...) of a (H, W, 4) / (B, H, W, 4) image of 4-channel pixels."""
     assert a.flags.c_contiguous or (a.ndim >= 2 and a.strides[-1] == a.itemsize)
-    h, w = a.shape[-2], a.shape[-1]
+    vec = dtype in capi.VECTOR_DTYPES
+    if vec:
+        assert a.shape[-1] == 4 and a.strides[-1] == a.itemsize
+    px = a.itemsize * (4 if vec else 1)      # bytes per pixel
+    h, w = (a.shape[-3], a.shape[-2]) if vec else (a.shape[-2], a.shape[-1])
+    nd = a.ndim - (1 if vec else 0)
     v = capi.View()
     v.data = a.ctypes.data
     v.dtype = CODES[a.dtype] if dtype is None else dtype
-    v.stride = a.strides[-2] // a.itemsize
+    v.stride = a.strides[-3 if vec else -2] // px
     v.img_width, v.img_height = w, h
     x, y, ww, hh = window if window is not None else (0, 0, w, h)
     v.offset_x, v.offset_y, v.width, v.height = x, y, ww, hh
-    v.batch = a.shape[0] if a.ndim == 3 else 1
-    v.plane_stride = a.strides[0] // a.itemsize if a.ndim == 3 else 0
+    v.batch = a.shape[0] if nd == 3 else 1
+    v.plane_stride = a.strides[0] // px if nd == 3 else 0
     return v
 
 
@@ -116,9 +123,10 @@ def _arr(values, np_dtype):
 def local(src: np.ndarray, dst: np.ndarray, mask_w: int, mask_h: int, *, weights=None, domain=None,
           acc_float=True, reduce=capi.SUM, boundary=capi.CLAMP, const_val=0.0,
           cvt=capi.CVT_TRUNC, post_div=0.0, fp_contract=True, src_window: Window = None,
-          dst_window: Window = None, dst_dtype=None, f64_out: Optional[np.ndarray] = None):
+          dst_window: Window = None, dst_dtype=None, f64_out: Optional[np.ndarray] = None,
+          src_dtype=None):
     d = capi.LocalDesc()
-    d.input = hview(src, src_window)
+    d.input = hview(src, src_window, src_dtype)
     d.output = hview(dst, dst_window, dst_dtype)
     d.mask_w, d.mask_h = mask_w, mask_h
     keep = []
@@ -171,6 +179,27 @@ def point(op, src0, dst, src1=None, *, alpha=1.0, beta=0.0, cvt=capi.CVT_TRUNC,
     d.output = hview(dst, windows[2], dst_dtype)
     d.op, d.alpha, d.beta, d.cvt = op, alpha, beta, cvt
     lib().oracle_point(ctypes.byref(d))
+
+
+def chain(src: np.ndarray, stages, *, src_window: Window = None) -> np.ndarray:
+    """The unfused chain of point and local stages (the stage dicts of
+    hipacc_amd.operators.prepare_fused, with numpy ``in1``): each stage an
+    oracle call over the previous stage's image; returns the last image."""
+    cur, win = src, src_window
+    shape = src.shape if src_window is None else src.shape[:-2] + (src_window[3], src_window[2])
+    for k, st in enumerate(stages):
+        st = dict(st)
+        kind = st.pop("kind")
+        nxt = np.zeros(shape, NP_DTYPES[st.pop("dtype")])
+        if kind == "point":
+            in1 = st.pop("in1", None)
+            point(st.pop("op", capi.POINT_AFFINE), cur, nxt, in1, alpha=st.pop("alpha", 1.0),
+                  beta=st.pop("beta", 0.0), cvt=st.pop("cvt", capi.CVT_TRUNC),
+                  windows=(win, st.pop("in1_window", None), None))
+        else:
+            local(cur, nxt, st.pop("mask_w"), st.pop("mask_h"), src_window=win, **st)
+        cur, win = nxt, None
+    return cur
 
 
 def reduce(src, mode, acc_dtype=None, window: Window = None, nthreads: int = 1):

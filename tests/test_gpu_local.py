@@ -29,8 +29,9 @@ def test_local_matches_reference_dsl(gpu, case, expected):
     spec = gc.local_spec(case)
     out_dtype = spec.pop("out_dtype")
     src = gc.input_for(case)
-    dst = empty_gpu(src.shape, out_dtype)
-    op.local(to_gpu(src), dst, dst_dtype=ob.CODES[np.dtype(out_dtype)], **spec)
+    dst = empty_gpu(expected.shape, out_dtype)
+    spec.setdefault("dst_dtype", ob.CODES[np.dtype(out_dtype)])
+    op.local(to_gpu(src), dst, **spec)
     np.testing.assert_array_equal(to_host(dst, out_dtype), expected)
 
 

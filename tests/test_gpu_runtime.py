@@ -105,6 +105,10 @@ def test_runtime_global_operators(results):
     assert abs(float(scal["fsum"][0]) - exact) <= 1e-5 * exact
     assert np.array_equal(_load(out, "hist_u32", np.uint32, (256,)),
                           np.bincount(u8.ravel(), minlength=256).astype(np.uint32))
+    # the generated-code names: hipaccApplyReductionShared / BinningSegmented
+    assert scal["fmax_shared"] == scal["fmax"]
+    assert np.array_equal(_load(out, "hist2_u32", np.uint32, (256,)),
+                          np.bincount(u8.ravel(), minlength=256).astype(np.uint32))
 
 
 @pytest.mark.gpu

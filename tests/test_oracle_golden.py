@@ -21,7 +21,7 @@ def test_oracle_matches_reference_dsl(case, expected):
     spec = gc.local_spec(case)
     if spec is not None:
         out_dtype = spec.pop("out_dtype")
-        dst = np.zeros((case["h"], case["w"]), dtype=out_dtype)
+        dst = np.zeros(expected.shape, dtype=out_dtype)
         ob.local(src, dst, **spec)
         np.testing.assert_array_equal(dst, expected)
     elif op == "bilat_f32":
@@ -56,5 +56,7 @@ def test_oracle_matches_reference_dsl(case, expected):
         dst = np.zeros(shape, src.dtype)
         ob.resample(src, dst, interp, **kw)
         np.testing.assert_array_equal(dst, expected)
+    elif op in ("fuse_u8", "fuse_f32"):
+        np.testing.assert_array_equal(ob.chain(src, gc.fuse_spec(case)), expected)
     else:
         pytest.fail(f"unmapped case {op}")
