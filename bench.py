@@ -235,23 +235,29 @@ def bench_config4(op, capi, dev, steps):
             "modes": out, "mpix_per_step": B * h * w / 1e6}
 
 
-def bench_fusion(op, capi, masks, dev, f32_in, f32_out, steps):
-    """Kernel fusion at 4096^2 float32: the fusion samples' Mixed chain
-    (point, point, 5x5 local, point, point, 5x5 local, point, point; points
-    v*1.0 + 0.25, locals the 5x5 Gaussian, CLAMP) and the P2L chain (point,
-    5x5 local) as one hipacc_hip_fused launch against the same chain as
-    separate launches through intermediate images.  Both are checked equal
-    before timing; bytes per pixel are the fused launch's (read in, write
-    out)."""
-    import torch
+def fusion_chains():
+    """The benchmarked fusion chains on float32 images: the fusion samples'
+    Mixed chain (point, point, 5x5 local, point, point, 5x5 local, point,
+    point) and P2L (point, 5x5 local); points v * 1.0 + 0.25, locals the 5x5
+    Gaussian (sigma 1), CLAMP."""
+    from hipacc_amd import capi, masks
     g5 = masks.gaussian_2d(5, 1.0)
     pt = dict(kind="point", dtype=capi.F32, op=capi.POINT_AFFINE, alpha=1.0, beta=0.25)
     lc = dict(kind="local", dtype=capi.F32, mask_w=5, mask_h=5, weights=g5,
               boundary=capi.CLAMP, fp_contract=True)
+    return {"mixed_pplpplpp_f32": [pt, pt, lc, pt, pt, lc, pt, pt], "p2l_pl_f32": [pt, lc]}
+
+
+def bench_fusion(op, capi, masks, dev, f32_in, f32_out, steps):
+    """Kernel fusion at 4096^2 float32: each chain of fusion_chains() as one
+    hipacc_hip_fused launch against the same chain as separate launches
+    through intermediate images.  Both are checked equal
+    before timing; bytes per pixel are the fused launch's (read in, write
+    out)."""
+    import torch
     tmp = [torch.empty_like(f32_in[0]) for _ in range(2)]
     out = {}
-    for name, chain in (("mixed_pplpplpp_f32", [pt, pt, lc, pt, pt, lc, pt, pt]),
-                        ("p2l_pl_f32", [pt, lc])):
+    for name, chain in fusion_chains().items():
         fused = [op.prepare_fused(f32_in[i], f32_out[i], chain) for i in range(len(f32_in))]
 
         def unfused_calls(i, chain=chain):
@@ -404,6 +410,10 @@ def profile_ops(op, capi, masks, dev, f32_in, f32_out):
                                              dst_dtype=capi.U32) for i in range(NSETS)],
         "reduce_sum_f32": [op.ReduceCall(f32_in[i], capi.SUM, res[i:i + 1]) for i in range(NSETS)],
         "histogram256_u8": [op.HistogramCall(u8_in[i], 256, bins[i]) for i in range(NSETS)],
+        "bilateral13_f32": [op.prepare_bilateral(f32_in[i], f32_out[i], 13, 3.0, 0.1,
+                                                 boundary=capi.MIRROR) for i in range(NSETS)],
+        "fused_mixed_f32": [op.prepare_fused(f32_in[i], f32_out[i], fusion_chains()["mixed_pplpplpp_f32"])
+                            for i in range(NSETS)],
     }
     for calls in fams.values():
         for i in range(10):
@@ -489,8 +499,9 @@ def main() -> int:
     ap.add_argument("--quick", action="store_true", help="headline operator only")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--profile", action="store_true",
-                    help="launch the headline kernel 20 times and a 64 MiB copy 20 times, "
-                         "for rocprofv3 counter runs; prints no result line")
+                    help="launch the headline kernel 20 times and a 64 MiB copy 20 times "
+                         "(both over 4 rotating buffer pairs), for rocprofv3 counter runs; "
+                         "prints no result line")
     ap.add_argument("--profile-ops", action="store_true",
                     help="with --profile: also launch the other benchmarked kernels 10 times "
                          "each (u8 ring, separable rings, reduction, histogram, bilateral, "
@@ -523,12 +534,13 @@ def main() -> int:
         calls = [c.with_variant(v, p) for c in head]
         for i in range(20):
             calls[i % NSETS].launch()
-        src = torch.empty(16 << 20, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(20):
-            dst.copy_(src)
+        # the copy rotates over the headline's own 4 buffer pairs (512 MiB),
+        # so that it too reads from HBM rather than the Infinity Cache
+        for i in range(20):
+            f32_out[i % NSETS].copy_(f32_in[i % NSETS])
         torch.cuda.synchronize()
-        print(f"profile run: 20 x {args.variant} 5x5 f32 4096^2, 20 x 64 MiB copy", flush=True)
+        print(f"profile run: 20 x {args.variant} 5x5 f32 4096^2, 20 x 64 MiB copy over 4 buffer "
+              "pairs", flush=True)
         if args.profile_ops:
             profile_ops(op, capi, masks, dev, f32_in, f32_out)
         return 0

@@ -1,0 +1,388 @@
+"""HIP emitter for Hipacc DSL programs: the `-emit-hip` target.
+
+The reference's source-to-source compiler rewrites a DSL program's host code
+into calls of its CUDA runtime and generates one kernel per Kernel class
+(lib/Rewrite/CreateHostStrings.cpp:650-1198, lib/Backend/CUDA.cpp:38-190).
+Here the kernels already exist in libhipacc_gfx950.so, so the emitter only
+has to recognise which operator each Kernel class is and rewrite main():
+
+  Image<T> img(w, h[, host])          -> HipaccImageHip<T> img = hipaccCreateMemory<T>(host, w, h)
+  Mask<T> m(coef)                     -> HipaccMask<T> m(sx, sy, &coef[0][0])
+  Domain d(sx, sy) / Domain d(mask)   -> HipaccMask<float> d(sx, sy) / a copy with holes
+  BoundaryCondition<T> b(img, m, MODE[, v])   (recorded; applied at the launch)
+  Accessor<T> a(b | img[, w, h, x, y])        -> HipaccAccessor<T> a = hipaccMakeAccessor(img, ...)
+  IterationSpace<T> is(img[, w, h, x, y])     -> HipaccAccessor<T> is = hipaccMakeAccessor(img, ...)
+  K k(is, a, m, ...)                  (recorded)
+  k.execute()                         -> hipaccLaunchLocal / hipaccLaunchPoint / hipaccApplyReduction
+  k.reduced_data()                    -> the reduction's value
+  img.data()                          -> hipaccReadMemory<T>(img)
+
+Kernel classes are parsed with libclang (hipacc_amd/emit/clang.py) and
+matched by hipacc_amd/emit/recognise.py; a kernel outside the recognised
+forms stops the emitter with the reason.  Usage:
+
+    python -m hipacc_amd.emit prog.cpp -o prog_hip.cpp -I <dsl include dir> [--no-fp-contract]
+"""
+from __future__ import annotations
+
+import re
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+from . import clang as C
+from .recognise import (GlobalOp, LocalOp, PointOp, Unsupported, classify_kernel,
+                        classify_reduce)
+
+
+class EmitError(RuntimeError):
+    pass
+
+
+@dataclass
+class KernelClass:
+    name: str
+    out_type: str
+    fields: Dict[str, Tuple[str, str]] = field(default_factory=dict)  # name -> (kind, elem type)
+    params: List[str] = field(default_factory=list)                  # constructor parameters
+    param_field: Dict[str, str] = field(default_factory=dict)         # parameter -> member
+    op: object = None
+    global_reduce: Optional[str] = None
+    extent: Tuple[int, int] = (0, 0)
+
+
+_TPL = re.compile(r"^(?:const\s+)?(?:hipacc::)?(\w+)\s*(?:<\s*(.+?)\s*>)?\s*&?$")
+
+
+def _tpl(type_spelling: str) -> Tuple[str, str]:
+    m = _TPL.match(type_spelling.strip())
+    if not m:
+        return "", ""
+    return m.group(1), (m.group(2) or "")
+
+
+def _args(tokens: Sequence[str], start: int) -> Tuple[List[str], int]:
+    """Top-level comma-separated arguments of the parenthesis at tokens[start]."""
+    assert tokens[start] == "("
+    depth, cur, out = 0, [], []
+    for j in range(start, len(tokens)):
+        t = tokens[j]
+        if t in "([{":
+            depth += 1
+            if depth == 1:
+                continue
+        elif t in ")]}":
+            depth -= 1
+            if depth == 0:
+                if cur:
+                    out.append(_join(cur))
+                return out, j
+        if depth == 1 and t == ",":
+            out.append(_join(cur))
+            cur = []
+        else:
+            cur.append(t)
+    raise EmitError("unbalanced parentheses")
+
+
+def _join(toks: Sequence[str]) -> str:
+    s = ""
+    for t in toks:
+        if s and (re.match(r"\w", t) and re.match(r".*\w$", s)):
+            s += " "
+        s += t
+    return s
+
+
+class Emitter:
+    def __init__(self, path: str, include_dirs: Sequence[str] = (), defines: Sequence[str] = (),
+                 fp_contract: bool = True):
+        self.path = path
+        self.fp_contract = fp_contract
+        args = ["-x", "c++", "-std=c++14"]
+        rd = C.resource_dir()
+        if rd:
+            args += ["-resource-dir", rd]
+        args += [f"-I{d}" for d in include_dirs] + [f"-D{d}" for d in defines]
+        self.tu = C.TranslationUnit(path, args)
+        with open(path, "rb") as f:
+            self.src = f.read()
+        main_errors = [e for e in self.tu.errors if e.startswith(path)]
+        if main_errors:
+            raise EmitError("the program does not parse:\n" + "\n".join(main_errors))
+        self.kernels: Dict[str, KernelClass] = {}
+        self.var_types: Dict[str, str] = {}
+        self.edits: List[Tuple[int, int, str]] = []
+
+    # ---- kernel classes ---------------------------------------------------------
+    def _kernel_class(self, c: C.Cursor) -> Optional[KernelClass]:
+        base = [ch for ch in c.children() if ch.kind == "C++ base class specifier"]
+        if len(base) != 1:
+            return None
+        name, targ = _tpl(base[0].type)
+        if name != "Kernel":
+            return None
+        k = KernelClass(c.spelling, targ.split(",")[0].strip(), extent=c.extent())
+        kernel_body = reduce_body = None
+        reduce_params: Tuple[str, str] = ("", "")
+        for ch in c.children():
+            if ch.kind == "FieldDecl":
+                kind, elem = _tpl(ch.type)
+                k.fields[ch.spelling] = (kind, elem)
+            elif ch.kind == "CXXConstructor":
+                sub = ch.children()
+                k.params = [p.spelling for p in sub if p.kind == "ParmDecl"]
+                for a, b in zip(sub, sub[1:]):
+                    if a.kind == "MemberRef" and b.kind == "DeclRefExpr":
+                        k.param_field[b.spelling] = a.spelling
+            elif ch.kind == "CXXMethod" and ch.spelling == "kernel":
+                body = [s for s in ch.children() if s.kind == "CompoundStmt"]
+                kernel_body = body[0].tokens() if body else None
+            elif ch.kind == "CXXMethod" and ch.spelling == "reduce":
+                ps = [p.spelling for p in ch.children() if p.kind == "ParmDecl"]
+                body = [s for s in ch.children() if s.kind == "CompoundStmt"]
+                if len(ps) == 2 and body:
+                    reduce_body, reduce_params = body[0].tokens(), (ps[0], ps[1])
+        if kernel_body is None:
+            raise EmitError(f"kernel class {k.name}: no kernel() body")
+        acc = [n for n, (kd, _) in k.fields.items() if kd == "Accessor"]
+        msk = [n for n, (kd, _) in k.fields.items() if kd == "Mask"]
+        dom = [n for n, (kd, _) in k.fields.items() if kd == "Domain"]
+        try:
+            k.op = classify_kernel(kernel_body, acc, msk, dom)
+            if reduce_body is not None:
+                k.global_reduce = classify_reduce(reduce_body, reduce_params)
+        except Unsupported as e:
+            raise EmitError(f"kernel class {k.name}: {e}") from None
+        return k
+
+    # ---- main() -----------------------------------------------------------------------
+    def _collect_var_types(self, c: C.Cursor) -> None:
+        for ch in c.walk():
+            if ch.kind == "VarDecl" and ch.in_main_file():
+                self.var_types[ch.spelling] = ch.type
+
+    def run(self) -> str:
+        main = None
+        for c in self.tu.cursor.children():
+            if not c.in_main_file():
+                continue
+            if c.kind in ("ClassDecl", "StructDecl"):
+                k = self._kernel_class(c)
+                if k:
+                    self.kernels[k.name] = k
+            elif c.kind == "FunctionDecl" and c.spelling == "main":
+                if any(ch.kind == "CompoundStmt" for ch in c.children()):
+                    main = c
+            elif c.kind == "VarDecl":
+                self.var_types[c.spelling] = c.type
+        if main is None:
+            raise EmitError("no main() definition")
+        self._collect_var_types(main)
+        for k in self.kernels.values():
+            b, e = k.extent
+            if self.src[e:e + 1] == b";":
+                e += 1
+            self.edits.append((b, e, f"// Kernel class {k.name}: {_describe(k)}"))
+        body = [ch for ch in main.children() if ch.kind == "CompoundStmt"][0]
+        self.images: Dict[str, str] = {}
+        self.bcs: Dict[str, Tuple[str, str, str]] = {}
+        self.accessors: Dict[str, Tuple[str, str, Optional[str]]] = {}
+        self.instances: Dict[str, Tuple[str, List[str]]] = {}
+        self.masks: Dict[str, str] = {}
+        self.timing = b"hipacc_last_kernel_timing" in self.src
+        for st in body.children():
+            self._statement(st)
+        self._inline_calls(main)
+        b0 = body.extent()[0]
+        init = "\n    hipaccInitHIP(0);" + ("\n    hipaccSetTiming(true);" if self.timing else "")
+        self.edits.append((b0 + 1, b0 + 1, init))
+        return self._apply()
+
+    def _statement(self, st: C.Cursor) -> None:
+        toks = st.tokens()
+        b, e = st.extent()
+        if self.src[e:e + 1] == b";":
+            e += 1
+        if st.kind == "DeclStmt":
+            decls = [ch for ch in st.children() if ch.kind == "VarDecl"]
+            if len(decls) != 1:
+                return
+            v = decls[0]
+            name, ty = v.spelling, v.type
+            kind, targ = _tpl(ty)
+            i = toks.index(name) if name in toks else -1
+            args = _args(toks, i + 1)[0] if 0 <= i < len(toks) - 1 and toks[i + 1] == "(" else []
+            rep = self._declaration(name, kind, targ, args)
+            if rep is not None:
+                self.edits.append((b, e, rep))
+        elif st.kind == "CallExpr":
+            m = re.match(r"^(\w+)\.execute\(\)$", "".join(toks).rstrip(";"))
+            if m and m.group(1) in self.instances:
+                self.edits.append((b, e, self._launch(m.group(1))))
+
+    def _declaration(self, name: str, kind: str, targ: str, args: List[str]) -> Optional[str]:
+        if kind == "Image":
+            self.images[name] = targ
+            host = args[2] if len(args) > 2 else "nullptr"
+            if len(args) < 2:
+                raise EmitError(f"Image {name}: expected (width, height[, data])")
+            return f"HipaccImageHip<{targ}> {name} = hipaccCreateMemory<{targ}>({host}, {args[0]}, {args[1]});"
+        if kind == "Mask":
+            if len(args) != 1:
+                raise EmitError(f"Mask {name}: expected one coefficient array")
+            dims = re.findall(r"\[(\d+)\]", self.var_types.get(args[0], ""))
+            if len(dims) != 2:
+                raise EmitError(f"Mask {name}: the size of {args[0]} is not a constant 2D array")
+            self.masks[name] = targ
+            sy, sx = dims
+            return f"HipaccMask<{targ}> {name}({sx}, {sy}, &{args[0]}[0][0]);"
+        if kind == "Domain":
+            if len(args) == 2:
+                self.masks[name] = "float"
+                return f"HipaccMask<float> {name}({args[0]}, {args[1]});"
+            if len(args) == 1 and args[0] in self.masks:
+                self.masks[name] = self.masks[args[0]]
+                return f"auto {name} = {args[0]};\n    {name}.domain_from_coef();"
+            raise EmitError(f"Domain {name}: expected (size_x, size_y) or (mask)")
+        if kind == "BoundaryCondition":
+            mode = [a for a in args if re.match(r"^(?:hipacc::)?Boundary::\w+$", a)]
+            if not args or len(mode) != 1:
+                raise EmitError(f"BoundaryCondition {name}: no Boundary mode")
+            mi = args.index(mode[0])
+            value = args[mi + 1] if len(args) > mi + 1 else "0"
+            self.bcs[name] = (args[0], mode[0].split("::")[-1], value)
+            return f"// BoundaryCondition {name}: {mode[0].split('::')[-1]} of {args[0]}"
+        if kind in ("Accessor", "IterationSpace"):
+            if not args:
+                raise EmitError(f"{kind} {name}: no image")
+            if any("Interpolate" in a for a in args):
+                raise EmitError(f"{kind} {name}: interpolating accessors are launched as resampling "
+                                "operators (hipacc_hip_resample), not emitted")
+            src = args[0]
+            if src in self.bcs:
+                img, mode, value = self.bcs[src]
+            else:
+                img, mode, value = src, "UNDEFINED", "0"
+            if img not in self.images:
+                raise EmitError(f"{kind} {name}: unknown image {img}")
+            self.accessors[name] = (img, mode, value)
+            crop = "" if len(args) == 1 else ", " + ", ".join(args[1:5])
+            if len(args) not in (1, 5):
+                raise EmitError(f"{kind} {name}: expected (image) or (image, w, h, x, y)")
+            return f"HipaccAccessor<{targ}> {name} = hipaccMakeAccessor({img}{crop});"
+        if kind in self.kernels:
+            self.instances[name] = (kind, args)
+            return f"// {kind} {name}({', '.join(args)}): launched at {name}.execute()"
+        return None
+
+    def _binding(self, inst: str) -> Tuple[KernelClass, Dict[str, str], str]:
+        kname, args = self.instances[inst]
+        k = self.kernels[kname]
+        if len(args) != len(k.params):
+            raise EmitError(f"{inst}: {len(args)} arguments for {len(k.params)} parameters")
+        bind = {}
+        for p, a in zip(k.params, args):
+            if p in k.param_field:
+                bind[k.param_field[p]] = a
+        iter_name = args[0]
+        if iter_name not in self.accessors:
+            raise EmitError(f"{inst}: the first argument is not an IterationSpace")
+        return k, bind, iter_name
+
+    def _launch(self, inst: str) -> str:
+        k, bind, it = self._binding(inst)
+        op = k.op
+        if k.global_reduce:
+            acc = bind[op.inputs[0]] if isinstance(op, PointOp) else None
+            if not isinstance(op, PointOp) or op.op != "COPY" or acc is None:
+                raise EmitError(f"{inst}: a global reduction's kernel() must copy its accessor")
+            return (f"hipaccLaunchPoint(HIPACC_POINT_COPY, {acc}, {it});\n"
+                    f"    {k.out_type} {inst}_reduced = hipaccApplyReduction<{k.out_type}>"
+                    f"({acc}, HipaccReduce::{k.global_reduce});")
+        if isinstance(op, PointOp):
+            ins = [bind[a] for a in op.inputs]
+            if op.op in ("COPY", "AFFINE"):
+                t0 = self.images[self.accessors[ins[0]][0]]
+                ab = ("" if op.op == "COPY" else
+                      f", static_cast<const HipaccAccessor<{t0}>*>(nullptr), {op.alpha}, {op.beta}")
+                return f"hipaccLaunchPoint(HIPACC_POINT_{op.op}, {ins[0]}, {it}{ab});"
+            return f"hipaccLaunchPoint(HIPACC_POINT_{op.op}, {ins[0]}, {it}, &{ins[1]});"
+        assert isinstance(op, LocalOp)
+        acc = bind[op.accessor]
+        img, mode, value = self.accessors[acc]
+        mtype = "float" if op.acc_type in ("float", "double") else "int"
+        o = f"{inst}_opt"
+        lines = [f"HipaccLocalOptions {o};",
+                 f"{o}.reduce = HipaccReduce::{op.reduce};",
+                 f"{o}.boundary = HipaccBoundary::{mode};"]
+        if mode == "CONSTANT":
+            lines.append(f"{o}.const_val = {value};")
+        lines.append(f"{o}.use_weights = {'true' if op.mask else 'false'};")
+        if op.round:
+            lines.append(f"{o}.round = true;")
+        if op.post_div:
+            lines.append(f"{o}.post_div = {op.post_div};")
+        lines.append(f"{o}.fp_contract = {'true' if self.fp_contract else 'false'};")
+        m = f"{inst}_mask"
+        if op.mask and op.domain:
+            lines.append(f"auto {m} = hipaccMaskCast<{mtype}>({bind[op.mask]});")
+            lines.append(f"{m}.domain = {bind[op.domain]}.domain;")
+        else:
+            lines.append(f"auto {m} = hipaccMaskCast<{mtype}>({bind[op.mask or op.domain]});")
+        lines.append(f"hipaccLaunchLocal({acc}, {it}, {m}, {o});")
+        return "{\n        " + "\n        ".join(lines) + "\n    }"
+
+    def _inline_calls(self, main: C.Cursor) -> None:
+        """img.data() and k.reduced_data() anywhere in main()."""
+        b, e = main.extent()
+        text = self.src[b:e].decode()
+        for m in re.finditer(r"\b(\w+)\s*\.\s*(data|reduced_data)\s*\(\s*\)", text):
+            name, what = m.group(1), m.group(2)
+            if what == "data" and name in self.images:
+                rep = f"hipaccReadMemory<{self.images[name]}>({name})"
+            elif what == "reduced_data" and name in self.instances:
+                rep = f"{name}_reduced"
+            else:
+                continue
+            s0 = b + len(text[:m.start()].encode())
+            s1 = b + len(text[:m.end()].encode())
+            if not any(x0 <= s0 < x1 for x0, x1, _ in self.edits):
+                self.edits.append((s0, s1, rep))
+
+    def _apply(self) -> str:
+        out = self.src
+        for b, e, rep in sorted(self.edits, key=lambda t: t[0], reverse=True):
+            out = out[:b] + rep.encode() + out[e:]
+        text = out.decode()
+        prologue = ("#include \"hipacc_hip.hpp\"\n"
+                    "// DSL names kept by the emitted host code\n"
+                    "typedef unsigned char uchar;\n"
+                    "typedef unsigned short ushort;\n"
+                    "typedef unsigned int uint;\n"
+                    "#ifndef HIPACC_CODEGEN\n#define HIPACC_CODEGEN\n#endif\n"
+                    "namespace hipacc { namespace math { using std::min; using std::max; } }")
+        text, n = re.subn(r'#include\s*"hipacc\.hpp"', prologue, text, count=1)
+        if n != 1:
+            raise EmitError('no #include "hipacc.hpp"')
+        header = ("// Emitted by hipacc_amd.emit from " + self.path.split("/")[-1] +
+                  " for the MI355X runtime (hipacc_amd/runtime/hipacc_hip.hpp).\n")
+        return header + text
+
+
+def _describe(k: KernelClass) -> str:
+    op = k.op
+    if k.global_reduce:
+        return f"global {k.global_reduce} reduction (hipaccApplyReduction)"
+    if isinstance(op, PointOp):
+        return f"point operator {op.op} (hipaccLaunchPoint)"
+    return (f"local operator {op.reduce} over {'mask ' + op.mask if op.mask else 'domain ' + op.domain}"
+            f"{', rounded' if op.round else ''} (hipaccLaunchLocal)")
+
+
+def emit(path: str, include_dirs: Sequence[str] = (), defines: Sequence[str] = (),
+         fp_contract: bool = True) -> str:
+    em = Emitter(path, include_dirs, defines, fp_contract)
+    try:
+        return em.run()
+    finally:
+        em.tu.close()

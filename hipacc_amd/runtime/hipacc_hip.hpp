@@ -239,6 +239,16 @@ inline void hipaccWriteDomainFromMask(HipaccMask<M>& mask, const M* host) {
   mask.domain_from_coef();
 }
 
+// The mask with its coefficients converted to To (the accumulator type of a
+// kernel's lambda: float or int) and the same holes.
+template <typename To, typename From>
+inline HipaccMask<To> hipaccMaskCast(const HipaccMask<From>& m) {
+  HipaccMask<To> r(m.size_x, m.size_y);
+  for (size_t i = 0; i < m.coef.size(); ++i) r.coef[i] = (To)m.coef[i];
+  r.domain = m.domain;
+  return r;
+}
+
 // Options of a local operator's launch.
 struct HipaccLocalOptions {
   HipaccReduce reduce = HipaccReduce::SUM;
