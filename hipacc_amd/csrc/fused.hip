@@ -297,6 +297,55 @@ __device__ __forceinline__ FastV<T> points_t(const FusedArgs& a, int from, int t
   return v;
 }
 
+// points_t over N values at once, stage by stage: each stage's parameters
+// and operator are decoded once per wave for all N values instead of once
+// per value.  xy(e) gives value e's pixel (inside the image: it is where a
+// second input is read).
+template <typename T, int OP, int N>
+__device__ __forceinline__ void point_op_vec(const FusedPoint& s, FastV<T> (&v)[N],
+                                             const FastV<T> (&u)[N]) {
+#pragma unroll
+  for (int e = 0; e < N; ++e) {
+    if constexpr (std::is_same<T, float>::value)
+      v[e] = point_pixel<float, true>(OP, v[e], u[e], (int32_t)v[e], (int32_t)u[e], s.alpha,
+                                      s.beta, s.cvt);
+    else
+      v[e] = (int32_t)point_pixel<T, false>(OP, (float)v[e], (float)u[e], v[e], u[e], s.alpha,
+                                            s.beta, s.cvt);
+  }
+}
+
+template <typename T, int N, typename XY>
+__device__ __forceinline__ void points_vec(const FusedArgs& a, int from, int to,
+                                           FastV<T> (&v)[N], XY&& xy, int b) {
+  for (int k = from; k < to; ++k) {
+    const FusedPoint& s = a.pt[k];
+    FastV<T> u[N];
+    if (s.in1.data) {
+      const T* p = static_cast<const T*>(s.in1.data) + (long long)b * s.in1.plane;
+#pragma unroll
+      for (int e = 0; e < N; ++e) {
+        const int2 q = xy(e);
+        u[e] = (FastV<T>)p[(long long)(s.in1.oy + q.y) * s.in1.stride + s.in1.ox + q.x];
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < N; ++e) u[e] = 0;
+    }
+    switch (s.op) {
+      case HIPACC_POINT_COPY: point_op_vec<T, HIPACC_POINT_COPY, N>(s, v, u); break;
+      case HIPACC_POINT_AFFINE: point_op_vec<T, HIPACC_POINT_AFFINE, N>(s, v, u); break;
+      case HIPACC_POINT_ADD: point_op_vec<T, HIPACC_POINT_ADD, N>(s, v, u); break;
+      case HIPACC_POINT_SUB: point_op_vec<T, HIPACC_POINT_SUB, N>(s, v, u); break;
+      case HIPACC_POINT_MUL: point_op_vec<T, HIPACC_POINT_MUL, N>(s, v, u); break;
+      case HIPACC_POINT_ABSDIFF: point_op_vec<T, HIPACC_POINT_ABSDIFF, N>(s, v, u); break;
+      case HIPACC_POINT_MIN: point_op_vec<T, HIPACC_POINT_MIN, N>(s, v, u); break;
+      case HIPACC_POINT_MAX: point_op_vec<T, HIPACC_POINT_MAX, N>(s, v, u); break;
+      default: point_op_vec<T, HIPACC_POINT_MAGNITUDE, N>(s, v, u); break;
+    }
+  }
+}
+
 // first_input for the fast path: the same pixel, as float.
 template <typename T>
 __device__ __forceinline__ float first_input_t(const FusedArgs& a, const T* in, int x, int y, int b) {
@@ -330,13 +379,15 @@ __device__ __forceinline__ void stage_tile(const FusedArgs& a, const T* in, floa
       const int r = i / W, c = i - r * W;
       v[k] = (FastV<T>)base[(long long)r * a.in_stride + c];
     }
+    points_vec<T, NPT>(a, a.seg[0][0], a.seg[0][1], v, [&](int k) {
+      const int i = min((int)threadIdx.x + 256 * k, N - 1);
+      const int r = i / W;
+      return make_int2(ox + i - r * W, oy + r);
+    }, b);
 #pragma unroll
     for (int k = 0; k < NPT; ++k) {
       const int i = threadIdx.x + 256 * k;
-      if (i < N) {
-        const int r = i / W, c = i - r * W;
-        tile[i] = (float)points_t<T>(a, a.seg[0][0], a.seg[0][1], v[k], ox + c, oy + r, b);
-      }
+      if (i < N) tile[i] = (float)v[k];
     }
   } else {
     for (int i = threadIdx.x; i < N; i += 256) {
@@ -439,14 +490,16 @@ __global__ __launch_bounds__(256) void fused_fast_kernel(const FusedArgs a) {
       // every pixel of the tile is the intermediate image's own pixel
       float acc[R2];
       conv_col<K0, R2, FMA>(t1, W1, lane, wave * R2, L0.w.f, acc);
+      FastV<T> v[R2];
+#pragma unroll
+      for (int j = 0; j < R2; ++j) v[j] = (FastV<T>)convert_f<T>(acc[j], L0.cvt, L0.post_div);
+      points_vec<T, R2>(a, a.seg[1][0], a.seg[1][1], v, [&](int j) {
+        return make_int2(ox2 + lane, oy2 + min(wave * R2 + j, H2 - 1));
+      }, b);
 #pragma unroll
       for (int j = 0; j < R2; ++j) {
         const int r = wave * R2 + j;
-        if (r < H2) {
-          const FastV<T> v = (FastV<T>)convert_f<T>(acc[j], L0.cvt, L0.post_div);
-          t2[r * W2 + lane] =
-              (float)points_t<T>(a, a.seg[1][0], a.seg[1][1], v, ox2 + lane, oy2 + r, b);
-        }
+        if (r < H2) t2[r * W2 + lane] = (float)v[j];
       }
     } else {
       for (int i = threadIdx.x; i < W2 * H2; i += 256) {
@@ -479,13 +532,16 @@ __global__ __launch_bounds__(256) void fused_fast_kernel(const FusedArgs a) {
   conv_col<KL, kFastRPW, FMA>(t2, W2, lane < TW ? lane : TW - 1, wave * kFastRPW, LL.w.f, acc);
   const int x = x0 + lane;
   if (lane >= TW || x >= a.w) return;
+  FastV<T> v[kFastRPW];
+#pragma unroll
+  for (int j = 0; j < kFastRPW; ++j) v[j] = (FastV<T>)convert_f<T>(acc[j], LL.cvt, LL.post_div);
+  points_vec<T, kFastRPW>(a, a.seg[2][0], a.seg[2][1], v, [&](int j) {
+    return make_int2(x, min(y0 + wave * kFastRPW + j, a.h - 1));
+  }, b);
 #pragma unroll
   for (int j = 0; j < kFastRPW; ++j) {
     const int y = y0 + wave * kFastRPW + j;
-    if (y >= a.h) break;
-    FastV<T> v = (FastV<T>)convert_f<T>(acc[j], LL.cvt, LL.post_div);
-    v = points_t<T>(a, a.seg[2][0], a.seg[2][1], v, x, y, b);
-    out[(long long)(a.out_oy + y) * a.out_stride + a.out_ox + x] = (T)v;
+    if (y < a.h) out[(long long)(a.out_oy + y) * a.out_stride + a.out_ox + x] = (T)v[j];
   }
 }
 
