@@ -17,46 +17,38 @@ namespace hh {
 
 // Neighbour terms landing on ring position i (outside the image), read by
 // the taps of in-image pixels p = i - o_t; the value read there is X(q),
-// q = bh(i).  A group of 16 lanes per ring position of one image: lane t
-// sums the taps of row t (rows >= ksize contribute 0), then the group folds
-// its row sums in a fixed butterfly order.
-constexpr int kBilatRingLanes = 16;
+// q = bh(i).  One thread per ring position of one image.
 __global__ __launch_bounds__(256) void bilat_ring_kernel(const BilatGradArgs a, float* ring,
                                                          long long rsize) {
-  const int j = (int)((blockIdx.x * 256u + threadIdx.x) / kBilatRingLanes);
-  const int ty = threadIdx.x & (kBilatRingLanes - 1);
+  const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
   const int b = blockIdx.y;
+  if (j >= rsize) return;
   const int r = a.ksize / 2;
+  int ix, iy;
+  ring_pos(a.w, a.h, r, r, j, ix, iy);
+  const int qx = bh_map(a.boundary, ix, 0, a.w, a.w), qy = bh_map(a.boundary, iy, 0, a.h, a.h);
+  const float* X = a.x + b * a.x_plane;
+  const float* Y = a.y + b * a.y_plane;
+  const float* D = a.d + b * a.d_plane;
+  const float* G = a.gy + b * a.gy_plane;
+  const float xq = X[(long long)qy * a.x_stride + qx];
   float acc = 0.0f;
-  if (j < rsize && ty < a.ksize) {
-    int ix, iy;
-    ring_pos32(a.w, a.h, r, r, j, ix, iy);
+  for (int ty = 0; ty < a.ksize; ++ty) {
     const int y = iy - (ty - r);
-    if (y >= 0 && y < a.h) {
-      const int qx = bh_map(a.boundary, ix, 0, a.w, a.w);
-      const int qy = bh_map(a.boundary, iy, 0, a.h, a.h);
-      const float* X = a.x + b * a.x_plane;
-      const float* Xr = X + (long long)y * a.x_stride;
-      const float* Yr = a.y + b * a.y_plane + (long long)y * a.y_stride;
-      const float* Dr = a.d + b * a.d_plane + (long long)y * a.d_stride;
-      const float* Gr = a.gy + b * a.gy_plane + (long long)y * a.gy_stride;
-      const float xq = X[(long long)qy * a.x_stride + qx];
-      for (int tx = 0; tx < a.ksize; ++tx) {
-        const int x = ix - (tx - r);
-        if (x < 0 || x >= a.w) continue;
-        const float xp = Xr[x];
-        const float yp = Yr[x];
-        const float ap = Gr[x] / Dr[x];
-        const float dd = xp - xq;
-        const float sv = __builtin_amdgcn_exp2f(a.k2 * (dd * dd)) * a.sp[ty * a.ksize + tx];
-        const float w = ap * sv;
-        acc += w + 2.0f * a.c * (w * dd) * (xq - yp);
-      }
+    if (y < 0 || y >= a.h) continue;
+    for (int tx = 0; tx < a.ksize; ++tx) {
+      const int x = ix - (tx - r);
+      if (x < 0 || x >= a.w) continue;
+      const float xp = X[(long long)y * a.x_stride + x];
+      const float yp = Y[(long long)y * a.y_stride + x];
+      const float ap = G[(long long)y * a.gy_stride + x] / D[(long long)y * a.d_stride + x];
+      const float dd = xp - xq;
+      const float sv = __builtin_amdgcn_exp2f(a.k2 * (dd * dd)) * a.sp[ty * a.ksize + tx];
+      const float w = ap * sv;
+      acc += w + 2.0f * a.c * (w * dd) * (xq - yp);
     }
   }
-#pragma unroll
-  for (int o = kBilatRingLanes / 2; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, kBilatRingLanes);
-  if (j < rsize && ty == 0) ring[b * rsize + j] = acc;
+  ring[b * rsize + j] = acc;
 }
 
 // 64 x 16 pixel tiles, 4 adjacent pixels per lane.  LDS holds, with a halo of
@@ -291,10 +283,8 @@ int launch_bilateral_grad(const BilatGradArgs& a0, float* dsigma, float sigma_r,
   if (st == HIPACC_OK && a.gx && ring && a.boundary != HIPACC_BOUNDARY_CONSTANT) {
     const int r = a.ksize / 2;
     const long long rs = ring_size(a.w, a.h, r, r);
-    if (rs * kBilatRingLanes >= (1ll << 31)) {
-      st = set_error(HIPACC_ERR_UNSUPPORTED, "bilateral_grad: %lld ring positions per image", rs);
-    } else if (rs > 0) {
-      dim3 grid((unsigned)((rs * kBilatRingLanes + 255) / 256), a.batch);
+    if (rs > 0) {
+      dim3 grid((unsigned)((rs + 255) / 256), a.batch);
       hipLaunchKernelGGL(bilat_ring_kernel, grid, dim3(256), 0, s, a, ring, rs);
       st = hip_status(hipGetLastError(), "bilat_ring_kernel launch");
       if (st == HIPACC_OK)

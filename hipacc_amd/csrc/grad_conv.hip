@@ -33,41 +33,33 @@ int launch_flip(const float* w, float* out, int n, hipStream_t s) {
   return hip_status(hipGetLastError(), "flip_kernel launch");
 }
 
-// A group of 16 lanes per ring position of one image: lane t of the group
-// sums tap row t (rows >= ky contribute 0), then the group folds its 16 row
-// sums in a fixed butterfly order.
-constexpr int kRingLanes = 16;
+// One thread per ring position of one image.
 __global__ __launch_bounds__(256) void conv_ring_kernel(const ConvGradArgs a, float* ring,
                                                         long long rsize) {
-  const int j = (int)((blockIdx.x * 256u + threadIdx.x) / kRingLanes);
-  const int ty = threadIdx.x & (kRingLanes - 1);
+  const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
   const int b = blockIdx.y;
+  if (j >= rsize) return;
   const int rx = a.kx / 2, ry = a.ky / 2;
+  int ix, iy;
+  ring_pos(a.w, a.h, rx, ry, j, ix, iy);
+  const float* G = a.gy + b * a.gy_plane;
   float e = 0.0f;
-  if (j < rsize && ty < a.ky) {
-    int ix, iy;
-    ring_pos32(a.w, a.h, rx, ry, j, ix, iy);
+  for (int ty = 0; ty < a.ky; ++ty) {
     const int y = iy + ry - ty;
-    if (y >= 0 && y < a.h) {
-      const float* G = a.gy + b * a.gy_plane + (long long)y * a.gy_stride;
-      for (int tx = 0; tx < a.kx; ++tx) {
-        const int x = ix + rx - tx;
-        if (x < 0 || x >= a.w) continue;
-        e = __builtin_fmaf(grad_weight(a, ty * a.kx + tx), G[x], e);
-      }
+    if (y < 0 || y >= a.h) continue;
+    for (int tx = 0; tx < a.kx; ++tx) {
+      const int x = ix + rx - tx;
+      if (x < 0 || x >= a.w) continue;
+      e = __builtin_fmaf(grad_weight(a, ty * a.kx + tx), G[(long long)y * a.gy_stride + x], e);
     }
   }
-#pragma unroll
-  for (int o = kRingLanes / 2; o >= 1; o >>= 1) e += __shfl_xor(e, o, kRingLanes);
-  if (j < rsize && ty == 0) ring[b * rsize + j] = e;
+  ring[b * rsize + j] = e;
 }
 
 int launch_conv_ring(const ConvGradArgs& a, float* ring, hipStream_t s) {
   const long long rs = ring_size(a.w, a.h, a.kx / 2, a.ky / 2);
   if (rs == 0) return HIPACC_OK;
-  if (rs * kRingLanes >= (1ll << 31))
-    return set_error(HIPACC_ERR_UNSUPPORTED, "local_grad: %lld ring positions per image", rs);
-  dim3 grid((unsigned)((rs * kRingLanes + 255) / 256), a.batch);
+  dim3 grid((unsigned)((rs + 255) / 256), a.batch);
   hipLaunchKernelGGL(conv_ring_kernel, grid, dim3(256), 0, s, a, ring, rs);
   return hip_status(hipGetLastError(), "conv_ring_kernel launch");
 }
@@ -77,32 +69,22 @@ __global__ __launch_bounds__(256) void fold_add_kernel(float* gx, long long gx_p
                                                        int gx_stride, int w, int h, int rx,
                                                        int ry, int mode, const float* ring,
                                                        long long rsize, long long bsize) {
-  const int j = (int)(blockIdx.x * 256u + threadIdx.x);
+  const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
   const int b = blockIdx.y;
   if (j >= bsize) return;
   int qx, qy;
-  band_pos32(w, h, rx, ry, j, qx, qy);
-  // Preimages of q per axis: q itself (k = -1), then the halo indices -r ..
-  // -1 and n .. n+r-1 (k = 0 .. 2r-1) that the mode maps onto q; the pairs
-  // are visited in that order, q x q skipped.  (No index arrays: dynamically
-  // indexed private arrays live in scratch memory.)
-  auto halo = [](int k, int r, int n) { return k < r ? k - r : n + (k - r); };
-  bool any = false;
-  for (int k = 0; k < 2 * rx && !any; ++k) any = bh_map(mode, halo(k, rx, w), 0, w, w) == qx;
-  for (int k = 0; k < 2 * ry && !any; ++k) any = bh_map(mode, halo(k, ry, h), 0, h, h) == qy;
-  if (!any) return;
+  band_pos(w, h, rx, ry, j, qx, qy);
+  int px[2 * (HIPACC_MAX_MASK_DIM / 2) + 1], py[2 * (HIPACC_MAX_MASK_DIM / 2) + 1];
+  const int nx = preimage(mode, qx, w, rx, px);
+  const int ny = preimage(mode, qy, h, ry, py);
+  if (nx == 1 && ny == 1) return;
   const float* R = ring + b * rsize;
   float acc = 0.0f;
-  for (int ky = -1; ky < 2 * ry; ++ky) {
-    const int iy = ky < 0 ? qy : halo(ky, ry, h);
-    if (ky >= 0 && bh_map(mode, iy, 0, h, h) != qy) continue;
-    for (int kx = -1; kx < 2 * rx; ++kx) {
-      const int ix = kx < 0 ? qx : halo(kx, rx, w);
-      if (kx >= 0 && bh_map(mode, ix, 0, w, w) != qx) continue;
-      if (kx < 0 && ky < 0) continue;
-      acc += R[ring_index(w, h, rx, ry, ix, iy)];
+  for (int jy = 0; jy < ny; ++jy)
+    for (int jx = 0; jx < nx; ++jx) {
+      if (jx == 0 && jy == 0) continue;
+      acc += R[ring_index(w, h, rx, ry, px[jx], py[jy])];
     }
-  }
   float* dst = gx + b * gx_plane + (long long)qy * gx_stride + qx;
   *dst = *dst + acc;
 }
@@ -112,8 +94,6 @@ int launch_fold_add(float* gx, long long gx_plane, int gx_stride, int w, int h, 
   if (mode == HIPACC_BOUNDARY_CONSTANT) return HIPACC_OK;
   const long long bs = band_size(w, h, rx, ry), rs = ring_size(w, h, rx, ry);
   if (bs == 0) return HIPACC_OK;
-  if (bs >= (1ll << 31))
-    return set_error(HIPACC_ERR_UNSUPPORTED, "fold_add: %lld band pixels per image", bs);
   dim3 grid((unsigned)((bs + 255) / 256), batch);
   hipLaunchKernelGGL(fold_add_kernel, grid, dim3(256), 0, s, gx, gx_plane, gx_stride, w, h, rx,
                      ry, mode, ring, rs, bs);

@@ -59,26 +59,6 @@ HH_HD void ring_pos(int w, int h, int rx, int ry, long long j, int& ix, int& iy)
   ix = right ? w + c : c - rx;
 }
 
-// ring_pos for per-image ring indices below 2^31 (kernels: 32-bit integer
-// division is a short instruction sequence, 64-bit division a long
-// software routine).
-HH_HD void ring_pos32(int w, int h, int rx, int ry, int j, int& ix, int& iy) {
-  const int ew = w + 2 * rx;
-  if (j < 2 * ry * ew) {
-    const int r = j / ew;
-    ix = j - r * ew - rx;
-    iy = r < ry ? r - ry : h + (r - ry);
-    return;
-  }
-  j -= 2 * ry * ew;
-  const int side = h * rx;
-  const bool right = j >= side;
-  if (right) j -= side;
-  iy = j / rx;
-  const int c = j - iy * rx;
-  ix = right ? w + c : c - rx;
-}
-
 // Pixels within (rx, ry) of an edge: full rows at the top and bottom, the
 // side columns in between.
 HH_HD void band_rows(int w, int h, int rx, int ry, int& top, int& bot0, int& side) {
@@ -107,24 +87,6 @@ HH_HD void band_pos(int w, int h, int rx, int ry, long long j, int& x, int& y) {
   const int lcols = side, rcols = w - rstart, per = lcols + rcols;
   y = top + (int)(j / per);
   const int c = (int)(j - (long long)(y - top) * per);
-  x = c < lcols ? c : rstart + (c - lcols);
-}
-// band_pos for per-image band indices below 2^31 (32-bit division).
-HH_HD void band_pos32(int w, int h, int rx, int ry, int j, int& x, int& y) {
-  int top, bot0, side;
-  band_rows(w, h, rx, ry, top, bot0, side);
-  const int full = (top + (h - bot0)) * w;
-  if (j < full) {
-    const int r = j / w;
-    x = j - r * w;
-    y = r < top ? r : bot0 + (r - top);
-    return;
-  }
-  j -= full;
-  const int rstart = w - rx > side ? w - rx : side;
-  const int lcols = side, rcols = w - rstart, per = lcols + rcols;
-  y = top + j / per;
-  const int c = j - (y - top) * per;
   x = c < lcols ? c : rstart + (c - lcols);
 }
 
