@@ -14,6 +14,8 @@
 // reflection, REPEAT: the wrap; CONSTANT: none) -- is evaluated once per
 // ring position here and added to the border band.  dw: per-block partial
 // sums, then a fixed-order reduction (deterministic run to run).
+#include <algorithm>
+
 #include "hh_grad.hpp"
 #include "hh_stream.hpp"
 
@@ -266,9 +268,9 @@ __global__ __launch_bounds__(256) void conv_dw_kernel(const ConvGradArgs a, int 
 // direct global->LDS loads (waits counted per wave, no barrier); waves that
 // touch the image border read X through the boundary map (per-lane column
 // tables, scalar row remap) one row ahead in registers.
-template <int R, int S, int PF>
+template <int R, int PF>
 __global__ __launch_bounds__(64) void conv_dw_stream_kernel(const ConvGradArgs a, int strips,
-                                                            int segs, int total_waves) {
+                                                            int segs, int total_waves, int S) {
   constexpr int K = 2 * R + 1, KK = K * K, V = 4;
   constexpr int HL = (R + V - 1) / V;
   constexpr int NW = V + 2 * R;
@@ -472,16 +474,24 @@ int launch_conv_dw(const ConvGradArgs& a0, float* dw, hipStream_t s) {
   const int kk = a.kx * a.ky;
   int nblocks;
   if (a.kx == a.ky && (a.kx == 3 || a.kx == 5 || a.kx == 7 || a.kx == 9) && a.dw_variant != kVarTile) {
-    constexpr int S = 32, PF = 8;
+    constexpr int PF = 8;
     const int strip = (64 - 2) * 4;   // HL = 1 for R <= 4
-    const int strips = ceil_div(a.w, strip), segs = ceil_div(a.h, S);
+    const int strips = ceil_div(a.w, strip);
+    // Segment heights such that all segments run in one round of the
+    // kGradBlocks resident waves (2 per SIMD at this kernel's registers): a
+    // second, partial round would double the time of the waves that take
+    // it (16 x 1024^2, 9x9: 32-row segments, 2560 waves, 111 us).
+    const long long per_seg_row = (long long)strips * a.batch;
+    int segs_fit = (int)std::max(1LL, (long long)kGradBlocks / per_seg_row);
+    const int S = std::max(ceil_div(a.h, segs_fit), 8);
+    const int segs = ceil_div(a.h, S);
     const int waves = strips * segs * a.batch;
     nblocks = min(waves, kGradBlocks);
     switch (a.kx) {
-      case 3: hipLaunchKernelGGL((conv_dw_stream_kernel<1, S, PF>), dim3(nblocks), dim3(64), 0, s, a, strips, segs, waves); break;
-      case 5: hipLaunchKernelGGL((conv_dw_stream_kernel<2, S, PF>), dim3(nblocks), dim3(64), 0, s, a, strips, segs, waves); break;
-      case 7: hipLaunchKernelGGL((conv_dw_stream_kernel<3, S, PF>), dim3(nblocks), dim3(64), 0, s, a, strips, segs, waves); break;
-      default: hipLaunchKernelGGL((conv_dw_stream_kernel<4, S, PF>), dim3(nblocks), dim3(64), 0, s, a, strips, segs, waves); break;
+      case 3: hipLaunchKernelGGL((conv_dw_stream_kernel<1, PF>), dim3(nblocks), dim3(64), 0, s, a, strips, segs, waves, S); break;
+      case 5: hipLaunchKernelGGL((conv_dw_stream_kernel<2, PF>), dim3(nblocks), dim3(64), 0, s, a, strips, segs, waves, S); break;
+      case 7: hipLaunchKernelGGL((conv_dw_stream_kernel<3, PF>), dim3(nblocks), dim3(64), 0, s, a, strips, segs, waves, S); break;
+      default: hipLaunchKernelGGL((conv_dw_stream_kernel<4, PF>), dim3(nblocks), dim3(64), 0, s, a, strips, segs, waves, S); break;
     }
   } else if (a.kx == a.ky && (a.kx == 3 || a.kx == 5 || a.kx == 7 || a.kx == 9)) {
     a.tiles_x = ceil_div(a.w, 64);
