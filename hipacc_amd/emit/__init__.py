@@ -190,8 +190,24 @@ class Emitter:
         self.instances: Dict[str, Tuple[str, List[str]]] = {}
         self.masks: Dict[str, str] = {}
         self.timing = b"hipacc_last_kernel_timing" in self.src
+        self.launches: List[Tuple[str, int, int]] = []
+        self.image_args: Dict[str, List[str]] = {}
+        self.iter_crop: Dict[str, bool] = {}
+        self.acc_src: Dict[str, str] = {}
         for st in body.children():
             self._statement(st)
+        mb, me = main.extent()
+        self.main_text = self.src[mb:me].decode()
+        for chain in self._chains():
+            if len(chain) == 1:
+                inst, b, e = chain[0]
+                self.edits.append((b, e, self._launch(inst)))
+                continue
+            for inst, b, e in chain[:-1]:
+                self.edits.append((b, e, f"// {inst}.execute(): fused into the launch of "
+                                         f"{chain[-1][0]}"))
+            inst, b, e = chain[-1]
+            self.edits.append((b, e, self._fused_launch([c[0] for c in chain])))
         self._inline_calls(main)
         b0 = body.extent()[0]
         init = "\n    hipaccInitHIP(0);" + ("\n    hipaccSetTiming(true);" if self.timing else "")
@@ -218,11 +234,12 @@ class Emitter:
         elif st.kind == "CallExpr":
             m = re.match(r"^(\w+)\.execute\(\)$", "".join(toks).rstrip(";"))
             if m and m.group(1) in self.instances:
-                self.edits.append((b, e, self._launch(m.group(1))))
+                self.launches.append((m.group(1), b, e))
 
     def _declaration(self, name: str, kind: str, targ: str, args: List[str]) -> Optional[str]:
         if kind == "Image":
             self.images[name] = targ
+            self.image_args[name] = args[:2]
             host = args[2] if len(args) > 2 else "nullptr"
             if len(args) < 2:
                 raise EmitError(f"Image {name}: expected (width, height[, data])")
@@ -266,6 +283,8 @@ class Emitter:
             if img not in self.images:
                 raise EmitError(f"{kind} {name}: unknown image {img}")
             self.accessors[name] = (img, mode, value)
+            self.acc_src[name] = src
+            self.iter_crop[name] = len(args) != 1
             crop = "" if len(args) == 1 else ", " + ", ".join(args[1:5])
             if len(args) not in (1, 5):
                 raise EmitError(f"{kind} {name}: expected (image) or (image, w, h, x, y)")
@@ -309,6 +328,13 @@ class Emitter:
             return f"hipaccLaunchPoint(HIPACC_POINT_{op.op}, {ins[0]}, {it}, &{ins[1]});"
         assert isinstance(op, LocalOp)
         acc = bind[op.accessor]
+        lines, o, m = self._local_setup(inst, op, bind)
+        lines.append(f"hipaccLaunchLocal({acc}, {it}, {m}, {o});")
+        return "{\n        " + "\n        ".join(lines) + "\n    }"
+
+    def _local_setup(self, inst: str, op: LocalOp, bind: Dict[str, str]):
+        """Declarations of a local launch's options and mask."""
+        acc = bind[op.accessor]
         img, mode, value = self.accessors[acc]
         mtype = "float" if op.acc_type in ("float", "double") else "int"
         o = f"{inst}_opt"
@@ -329,8 +355,88 @@ class Emitter:
             lines.append(f"{m}.domain = {bind[op.domain]}.domain;")
         else:
             lines.append(f"auto {m} = hipaccMaskCast<{mtype}>({bind[op.mask or op.domain]});")
-        lines.append(f"hipaccLaunchLocal({acc}, {it}, {m}, {o});")
-        return "{\n        " + "\n        ".join(lines) + "\n    }"
+        return lines, o, m
+
+    # ---- fusion ---------------------------------------------------------------------
+    _FUSED_TYPES = {"uchar", "unsigned char", "char", "signed char", "short", "ushort",
+                    "unsigned short", "int", "float"}
+
+    def _chain_input(self, inst: str) -> Optional[str]:
+        """The accessor a fusible kernel reads its chain value through."""
+        k, bind, _ = self._binding(inst)
+        if k.global_reduce or k.out_type not in self._FUSED_TYPES:
+            return None
+        if isinstance(k.op, LocalOp):
+            return bind[k.op.accessor]
+        if k.op.op in ("COPY", "AFFINE", "ADD", "SUB", "MUL"):
+            return bind[k.op.inputs[0]]
+        return None
+
+    def _fusible(self, prev: str, nxt: str) -> bool:
+        """Whether kernel nxt can take kernel prev's output image without the
+        image: nxt reads it through its chain input only, nothing else in
+        main() names it, and every image involved has the same size."""
+        a_prev, a_next = self._chain_input(prev), self._chain_input(nxt)
+        if a_prev is None or a_next is None:
+            return False
+        _, _, it = self._binding(prev)
+        x = self.accessors[it][0]
+        if self.accessors[a_next][0] != x or self.iter_crop.get(it) or self.iter_crop.get(a_next):
+            return False
+        if self.images.get(x) not in self._FUSED_TYPES:
+            return False
+        k, bind, it_next = self._binding(nxt)
+        for other in (k.op.inputs[1:] if isinstance(k.op, PointOp) else []):
+            if self.accessors[bind[other]][0] == x:
+                return False
+        sizes = {tuple(self.image_args[self.accessors[n][0]]) for n in (a_prev, it, a_next, it_next)}
+        if len(sizes) != 1:
+            return False
+        # the image, its iteration space and the accessor (and boundary
+        # condition) reading it are the only names of it in main()
+        decls = 1 + sum(1 for src in self.acc_src.values() if src == x)
+        decls += sum(1 for n, (img, _, _) in self.bcs.items() if img == x)
+        if len(re.findall(r"\b%s\b" % re.escape(x), self.main_text)) != decls:
+            return False
+        users = [i for i, (_, args) in self.instances.items() if a_next in args or it in args]
+        return sorted(users) == sorted({prev, nxt})
+
+    def _chains(self) -> List[List[Tuple[str, int, int]]]:
+        chains: List[List[Tuple[str, int, int]]] = []
+        for launch in self.launches:
+            if chains and self._extends(chains[-1], launch[0]):
+                chains[-1].append(launch)
+            else:
+                chains.append([launch])
+        return chains
+
+    def _extends(self, chain, inst: str) -> bool:
+        if len(chain) >= 8 or not self._fusible(chain[-1][0], inst):
+            return False
+        locals_ = sum(isinstance(self._binding(c[0])[0].op, LocalOp) for c in chain)
+        return locals_ + isinstance(self._binding(inst)[0].op, LocalOp) <= 2
+
+    def _fused_launch(self, insts: List[str]) -> str:
+        name = f"{insts[-1]}_chain"
+        lines = [f"std::vector<HipaccFusedStage> {name};"]
+        for inst in insts:
+            k, bind, _ = self._binding(inst)
+            op = k.op
+            if isinstance(op, LocalOp):
+                ll, o, m = self._local_setup(inst, op, bind)
+                lines += ll
+                lines.append(f"{name}.push_back(hipaccFusedLocal<{k.out_type}>({m}, {o}));")
+            elif op.op in ("COPY", "AFFINE"):
+                lines.append(f"{name}.push_back(hipaccFusedPoint<{k.out_type}>(HIPACC_POINT_{op.op}, "
+                             f"{op.alpha}, {op.beta}));")
+            else:
+                lines.append(f"{name}.push_back(hipaccFusedPoint2<{k.out_type}>("
+                             f"HIPACC_POINT_{op.op}, {bind[op.inputs[1]]}));")
+        first = self._chain_input(insts[0])
+        last_it = self._binding(insts[-1])[2]
+        lines.append(f"hipaccLaunchFused({first}, {last_it}, {name});")
+        return ("{\n        // fused: " + " -> ".join(insts) + "\n        " +
+                "\n        ".join(lines) + "\n    }")
 
     def _inline_calls(self, main: C.Cursor) -> None:
         """img.data() and k.reduced_data() anywhere in main()."""

@@ -188,6 +188,113 @@ int hipaccLaunchPoint(int op, const HipaccAccessor<T0>& in0, const HipaccAccesso
   return st;
 }
 
+// One stage of a fused chain (hipacc_hip_fused): a point operator on the
+// previous image (and, for a two-input op, on in1) or a local operator over
+// it, writing pixels of type dtype.
+struct HipaccFusedStage {
+  int kind = HIPACC_STAGE_POINT;
+  int dtype = HIPACC_F32;
+  int op = HIPACC_POINT_COPY;
+  float alpha = 1.0f, beta = 0.0f;
+  bool round = false;
+  hipacc_view in1{};
+  int mask_w = 0, mask_h = 0, acc_dtype = HIPACC_F32;
+  std::vector<float> wf;
+  std::vector<int32_t> wi;
+  std::vector<uint8_t> domain;
+  HipaccLocalOptions opt;
+};
+
+// A one-input point stage writing pixels of type TO.
+template <typename TO>
+HipaccFusedStage hipaccFusedPoint(int op, float alpha = 1.0f, float beta = 0.0f,
+                                  bool round = false) {
+  HipaccFusedStage s;
+  s.dtype = hipacc_pixel_code<TO>::value;
+  s.op = op;
+  s.alpha = alpha;
+  s.beta = beta;
+  s.round = round;
+  return s;
+}
+
+// A two-input point stage (ADD, SUB, ...) with in1 as its second operand.
+template <typename TO, typename T1>
+HipaccFusedStage hipaccFusedPoint2(int op, const HipaccAccessor<T1>& in1) {
+  HipaccFusedStage s = hipaccFusedPoint<TO>(op);
+  s.in1 = hipaccView(in1);
+  return s;
+}
+
+// A local stage writing pixels of type TO; the accumulator follows the mask.
+template <typename TO, typename M>
+HipaccFusedStage hipaccFusedLocal(const HipaccMask<M>& mask, const HipaccLocalOptions& opt) {
+  HipaccFusedStage s;
+  s.kind = HIPACC_STAGE_LOCAL;
+  s.dtype = hipacc_pixel_code<TO>::value;
+  s.mask_w = mask.size_x;
+  s.mask_h = mask.size_y;
+  s.acc_dtype = hipacc_mask_acc<M>::value;
+  if (s.acc_dtype == HIPACC_F32)
+    s.wf.assign(mask.coef.begin(), mask.coef.end());
+  else
+    s.wi.assign(mask.coef.begin(), mask.coef.end());
+  s.domain = mask.domain;
+  s.opt = opt;
+  return s;
+}
+
+// The chain of stages from `in` to `out` as one launch (the fused kernel of
+// lib/AST/ASTFuse.cpp:212-413): bit-identical to launching the stages one
+// by one through intermediate images of out's size.
+template <typename TI, typename TO>
+int hipaccLaunchFused(const HipaccAccessor<TI>& in, const HipaccAccessor<TO>& out,
+                      const std::vector<HipaccFusedStage>& stages,
+                      const HipaccExecutionParameterHip& ep = nullptr) {
+  hipacc_fused_desc d;
+  std::memset(&d, 0, sizeof(d));
+  hipacc_local_desc locals[HIPACC_MAX_FUSED_STAGES];
+  std::memset(locals, 0, sizeof(locals));
+  if (stages.empty() || stages.size() > HIPACC_MAX_FUSED_STAGES)
+    return hipaccCheck(HIPACC_ERR_INVALID_ARG, "hipaccLaunchFused: 1..8 stages");
+  d.in = hipaccView(in);
+  d.out = hipaccView(out);
+  d.num_stages = (int32_t)stages.size();
+  for (size_t k = 0; k < stages.size(); ++k) {
+    const HipaccFusedStage& st = stages[k];
+    hipacc_fused_stage& g = d.stages[k];
+    g.kind = st.kind;
+    g.dtype = st.dtype;
+    if (st.kind == HIPACC_STAGE_POINT) {
+      g.op = st.op;
+      g.alpha = st.alpha;
+      g.beta = st.beta;
+      g.cvt = st.round ? HIPACC_CVT_ROUND : HIPACC_CVT_TRUNC;
+      g.in1 = st.in1;
+    } else {
+      hipacc_local_desc& l = locals[k];
+      l.mask_w = st.mask_w;
+      l.mask_h = st.mask_h;
+      l.acc_dtype = st.acc_dtype;
+      l.weights_f32 = st.wf.empty() ? nullptr : st.wf.data();
+      l.weights_i32 = st.wi.empty() ? nullptr : st.wi.data();
+      l.domain = st.domain.data();
+      l.use_weights = st.opt.use_weights ? 1 : 0;
+      l.reduce = (int32_t)st.opt.reduce;
+      l.boundary = (int32_t)st.opt.boundary;
+      l.const_val = st.opt.const_val;
+      l.cvt = st.opt.round ? HIPACC_CVT_ROUND : HIPACC_CVT_TRUNC;
+      l.post_div = st.opt.post_div;
+      l.fp_contract = st.opt.fp_contract ? 1 : 0;
+      g.local = &l;
+    }
+  }
+  if (ep) ep->pre_kernel();
+  int st = hipaccCheck(hipacc_hip_fused(&d, hipaccStreamOf(ep)), "hipaccLaunchFused");
+  if (ep) ep->post_kernel();
+  return st;
+}
+
 // The bilateral filter kernel (range weight from sigma_r, spatial weight
 // from sigma_d or a host table).
 template <typename T>

@@ -105,3 +105,16 @@ def test_emitter_refuses_unknown_kernel(tmp_path):
     p.write_text(src)
     with pytest.raises(EmitError, match="Scale"):
         emit(str(p), [DSL_HEADERS])
+
+
+def test_fusible_chain_is_one_launch():
+    """The Mixed fusion sample's chain, whose intermediate images nothing else
+    names, is emitted as one hipaccLaunchFused; the point_f32 program's kernels,
+    whose images are all written out, stay separate launches."""
+    fused = (DSL / "emitted" / "fuse_mixed_u8.cpp").read_text()
+    assert fused.count("hipaccLaunchFused(") == 1
+    assert "hipaccLaunchLocal(" not in fused and "hipaccLaunchPoint(" not in fused
+    assert fused.count("push_back(") == 8
+    separate = (DSL / "emitted" / "point_f32.cpp").read_text()
+    assert "hipaccLaunchFused(" not in separate
+    assert separate.count("hipaccLaunchPoint(") == 3 and separate.count("hipaccLaunchLocal(") == 1
