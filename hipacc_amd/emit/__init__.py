@@ -39,6 +39,15 @@ class EmitError(RuntimeError):
 
 
 @dataclass
+class UserOp:
+    """A kernel() the library has no operator for: compiled as it is, as
+    device code against hipacc_hip_device.hpp."""
+    body: str
+    reason: str
+    kind: str = "user"
+
+
+@dataclass
 class KernelClass:
     name: str
     out_type: str
@@ -84,6 +93,28 @@ def _args(tokens: Sequence[str], start: int) -> Tuple[List[str], int]:
     raise EmitError("unbalanced parentheses")
 
 
+def _member_inits(toks: Sequence[str], params: Sequence[str]) -> Dict[str, str]:
+    """parameter -> member of a constructor's `member(parameter)` initialisers."""
+    depth, start = 0, None
+    for j, t in enumerate(toks):
+        if t == "(":
+            depth += 1
+        elif t == ")":
+            depth -= 1
+        elif t == ":" and depth == 0:
+            start = j + 1
+            break
+    out: Dict[str, str] = {}
+    if start is None:
+        return out
+    j = start
+    while j + 3 < len(toks) and toks[j] != "{":
+        if toks[j + 1] == "(" and toks[j + 3] == ")" and toks[j + 2] in params:
+            out[toks[j + 2]] = toks[j]
+        j += 1
+    return out
+
+
 def _join(toks: Sequence[str]) -> str:
     s = ""
     for t in toks:
@@ -110,6 +141,7 @@ class Emitter:
         if main_errors:
             raise EmitError("the program does not parse:\n" + "\n".join(main_errors))
         self.kernels: Dict[str, KernelClass] = {}
+        self.field_types: Dict[Tuple[str, str], str] = {}
         self.var_types: Dict[str, str] = {}
         self.edits: List[Tuple[int, int, str]] = []
 
@@ -123,20 +155,26 @@ class Emitter:
             return None
         k = KernelClass(c.spelling, targ.split(",")[0].strip(), extent=c.extent())
         kernel_body = reduce_body = None
+        kernel_text = "" 
         reduce_params: Tuple[str, str] = ("", "")
         for ch in c.children():
             if ch.kind == "FieldDecl":
                 kind, elem = _tpl(ch.type)
                 k.fields[ch.spelling] = (kind, elem)
+                self.field_types[(k.name, ch.spelling)] = ch.type
             elif ch.kind == "CXXConstructor":
                 sub = ch.children()
                 k.params = [p.spelling for p in sub if p.kind == "ParmDecl"]
-                for a, b in zip(sub, sub[1:]):
-                    if a.kind == "MemberRef" and b.kind == "DeclRefExpr":
-                        k.param_field[b.spelling] = a.spelling
+                k.param_field.update(_member_inits(ch.tokens(), k.params))
             elif ch.kind == "CXXMethod" and ch.spelling == "kernel":
                 body = [s for s in ch.children() if s.kind == "CompoundStmt"]
                 kernel_body = body[0].tokens() if body else None
+                if body:
+                    b0, b1 = body[0].extent()
+                    kernel_text = self.src[b0:b1].decode()
+            elif ch.kind == "CXXMethod" and ch.spelling not in ("reduce", "binning"):
+                raise EmitError(f"kernel class {k.name}: member function {ch.spelling}() "
+                                "is not emitted")
             elif ch.kind == "CXXMethod" and ch.spelling == "reduce":
                 ps = [p.spelling for p in ch.children() if p.kind == "ParmDecl"]
                 body = [s for s in ch.children() if s.kind == "CompoundStmt"]
@@ -148,11 +186,16 @@ class Emitter:
         msk = [n for n, (kd, _) in k.fields.items() if kd == "Mask"]
         dom = [n for n, (kd, _) in k.fields.items() if kd == "Domain"]
         try:
-            k.op = classify_kernel(kernel_body, acc, msk, dom)
             if reduce_body is not None:
                 k.global_reduce = classify_reduce(reduce_body, reduce_params)
         except Unsupported as e:
             raise EmitError(f"kernel class {k.name}: {e}") from None
+        try:
+            k.op = classify_kernel(kernel_body, acc, msk, dom)
+        except Unsupported as e:
+            if reduce_body is not None:
+                raise EmitError(f"kernel class {k.name}: {e}") from None
+            k.op = UserOp(kernel_text, str(e))
         return k
 
     # ---- main() -----------------------------------------------------------------------
@@ -182,7 +225,11 @@ class Emitter:
             b, e = k.extent
             if self.src[e:e + 1] == b";":
                 e += 1
-            self.edits.append((b, e, f"// Kernel class {k.name}: {_describe(k)}"))
+            text = f"// Kernel class {k.name}: {_describe(k)}"
+            if isinstance(k.op, UserOp):
+                text += "\n" + self._user_struct(k)
+            self.edits.append((b, e, text))
+        self.user = any(isinstance(k.op, UserOp) for k in self.kernels.values())
         body = [ch for ch in main.children() if ch.kind == "CompoundStmt"][0]
         self.images: Dict[str, str] = {}
         self.bcs: Dict[str, Tuple[str, str, str]] = {}
@@ -291,6 +338,8 @@ class Emitter:
             return f"HipaccAccessor<{targ}> {name} = hipaccMakeAccessor({img}{crop});"
         if kind in self.kernels:
             self.instances[name] = (kind, args)
+            if isinstance(self.kernels[kind].op, UserOp):
+                return self._user_instance(name)
             return f"// {kind} {name}({', '.join(args)}): launched at {name}.execute()"
         return None
 
@@ -318,6 +367,8 @@ class Emitter:
             return (f"hipaccLaunchPoint(HIPACC_POINT_COPY, {acc}, {it});\n"
                     f"    {k.out_type} {inst}_reduced = hipaccApplyReduction<{k.out_type}>"
                     f"({acc}, HipaccReduce::{k.global_reduce});")
+        if isinstance(op, UserOp):
+            return f"hipaccLaunchUser({inst}, {it});"
         if isinstance(op, PointOp):
             ins = [bind[a] for a in op.inputs]
             if op.op in ("COPY", "AFFINE"):
@@ -357,6 +408,47 @@ class Emitter:
             lines.append(f"auto {m} = hipaccMaskCast<{mtype}>({bind[op.mask or op.domain]});")
         return lines, o, m
 
+    # ---- user kernels -----------------------------------------------------------------
+    def _user_struct(self, k: KernelClass) -> str:
+        members, pos = [], ["it_x = x_;", "it_y = y_;"]
+        for name, (kind, elem) in k.fields.items():
+            if kind == "Accessor":
+                members.append(f"hipacc_dev::Accessor<{elem}> {name};")
+                pos.append(f"{name}.px = x_; {name}.py = y_;")
+            elif kind == "Mask":
+                members.append(f"hipacc_dev::Mask<{elem}> {name};")
+            elif kind == "Domain":
+                members.append(f"hipacc_dev::Domain {name};")
+            elif kind in ("IterationSpace", "Image", "BoundaryCondition"):
+                raise EmitError(f"kernel class {k.name}: member {name} of type {kind}")
+            else:
+                members.append(f"{self.field_types[(k.name, name)]} {name};")
+        lines = [f"struct {k.name} : hipacc_dev::Kernel<{k.out_type}> {{"]
+        lines += ["    " + m for m in members]
+        lines.append("    __device__ void set_pos(int x_, int y_) { " + " ".join(pos) + " }")
+        lines.append("    __device__ void kernel() " + k.op.body)
+        lines.append("};")
+        return "\n".join(lines)
+
+    def _user_instance(self, inst: str) -> str:
+        k, bind, _ = self._binding(inst)
+        out = [f"{k.name} {inst};"]
+        for name, (kind, _) in k.fields.items():
+            if name not in bind:
+                raise EmitError(f"{inst}: member {name} is not set by the constructor")
+            arg = bind[name]
+            if kind == "Accessor":
+                img, mode, value = self.accessors[arg]
+                out.append(f"{inst}.{name} = hipaccDevAccessor({arg}, HipaccBoundary::{mode}, "
+                           f"{value});")
+            elif kind == "Mask":
+                out.append(f"{inst}.{name} = hipaccDevMask({arg});")
+            elif kind == "Domain":
+                out.append(f"{inst}.{name} = hipaccDevDomain({arg});")
+            else:
+                out.append(f"{inst}.{name} = {arg};")
+        return "\n    ".join(out)
+
     # ---- fusion ---------------------------------------------------------------------
     _FUSED_TYPES = {"uchar", "unsigned char", "char", "signed char", "short", "ushort",
                     "unsigned short", "int", "float"}
@@ -364,7 +456,7 @@ class Emitter:
     def _chain_input(self, inst: str) -> Optional[str]:
         """The accessor a fusible kernel reads its chain value through."""
         k, bind, _ = self._binding(inst)
-        if k.global_reduce or k.out_type not in self._FUSED_TYPES:
+        if k.global_reduce or isinstance(k.op, UserOp) or k.out_type not in self._FUSED_TYPES:
             return None
         if isinstance(k.op, LocalOp):
             return bind[k.op.accessor]
@@ -460,13 +552,15 @@ class Emitter:
         for b, e, rep in sorted(self.edits, key=lambda t: t[0], reverse=True):
             out = out[:b] + rep.encode() + out[e:]
         text = out.decode()
-        prologue = ("#include \"hipacc_hip.hpp\"\n"
+        prologue = (("#include \"hipacc_hip_device.hpp\"\n" if self.user else
+                     "#include \"hipacc_hip.hpp\"\n") +
                     "// DSL names kept by the emitted host code\n"
                     "typedef unsigned char uchar;\n"
                     "typedef unsigned short ushort;\n"
                     "typedef unsigned int uint;\n"
                     "#ifndef HIPACC_CODEGEN\n#define HIPACC_CODEGEN\n#endif\n"
-                    "namespace hipacc { namespace math { using std::min; using std::max; } }")
+                    "namespace hipacc { namespace math { using std::min; using std::max; } }" +
+                    ("\nnamespace hipacc { using Reduce = hipacc_dev::Reduce; }" if self.user else ""))
         text, n = re.subn(r'#include\s*"hipacc\.hpp"', prologue, text, count=1)
         if n != 1:
             raise EmitError('no #include "hipacc.hpp"')
@@ -477,6 +571,8 @@ class Emitter:
 
 def _describe(k: KernelClass) -> str:
     op = k.op
+    if isinstance(op, UserOp):
+        return "device code, the kernel() body compiled as written (hipaccLaunchUser)"
     if k.global_reduce:
         return f"global {k.global_reduce} reduction (hipaccApplyReduction)"
     if isinstance(op, PointOp):

@@ -97,13 +97,25 @@ def test_emitter_reproduces_committed_output(name):
 
 
 @pytest.mark.skipif(not _clang_ok(), reason="libclang or the DSL headers are not available")
-def test_emitter_refuses_unknown_kernel(tmp_path):
+def test_unrecognised_kernel_becomes_device_code(tmp_path):
+    """A kernel() outside the library's operators is compiled as written
+    (a device struct launched by hipaccLaunchUser); a kernel class with
+    other member functions is refused."""
     from hipacc_amd.emit import EmitError, emit
     src = (DSL / "point_f32.cpp").read_text().replace(
-        "output() = input() * 0.5f + 0.25f;", "output() = sqrt(input());")
-    p = tmp_path / "bad.cpp"
+        "output() = input() * 0.5f + 0.25f;", "output() = sqrtf(input());")
+    p = tmp_path / "user.cpp"
     p.write_text(src)
-    with pytest.raises(EmitError, match="Scale"):
+    text = emit(str(p), [DSL_HEADERS])
+    assert "struct Scale : hipacc_dev::Kernel<float>" in text
+    assert "__device__ void kernel() { output() = sqrtf(input()); }" in text
+    assert "hipaccLaunchUser(scale, iscaled);" in text
+    assert '#include "hipacc_hip_device.hpp"' in text
+    bad = src.replace("    void kernel() { output() = sqrtf(input()); }",
+                      "    float half(float v) { return v * 0.5f; }\n"
+                      "    void kernel() { output() = half(input()); }")
+    p.write_text(bad)
+    with pytest.raises(EmitError, match="half"):
         emit(str(p), [DSL_HEADERS])
 
 

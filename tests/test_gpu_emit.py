@@ -4,8 +4,10 @@ programs on the reference's DSL host emulation: the digests in
 tests/golden/emit_golden.json (and, where oracle/_ref/dsl_<name> is built,
 that program's output, for the report of a mismatch).
 
-Bar: bit-exact for the images (the programs are emitted without
-contraction, the DSL host emulation's arithmetic); the float sum of the
+Bar: bit-exact for the images (the programs are emitted and compiled
+without contraction, the DSL host emulation's arithmetic), except programs
+calling transcendental functions in their own kernels (device expf against
+glibc's: within TOLERANCE of the DSL program's output); the float sum of the
 reduction within 1e-6 of the sum of magnitudes of the double sum (the
 emulation's sequential float sum differs from both by up to its own
 n * eps bound), its minimum and maximum exact.
@@ -22,6 +24,9 @@ pytestmark = pytest.mark.gpu
 
 ROOT = Path(__file__).resolve().parents[1]
 GOLDEN = json.loads((ROOT / "tests" / "golden" / "emit_golden.json").read_text())
+# Programs whose kernels call transcendental functions (device expf against
+# glibc's): pixel type and the largest difference allowed per pixel.
+TOLERANCE = {"bilateral_u8": (np.uint8, 1)}
 
 
 def _run(exe: Path, out: Path) -> bytes:
@@ -54,6 +59,15 @@ def test_emitted_program_matches_dsl(gpu, name, tmp_path):
     detail = ""
     if ref_exe.exists():
         ref = _run(ref_exe, tmp_path / "dsl.bin")
+        assert hashlib.sha256(ref).hexdigest() == g["sha256"], "the DSL program changed"
+        if name in TOLERANCE:
+            dt, tol = TOLERANCE[name]
+            a = np.frombuffer(data, dt).astype(np.int64)
+            b = np.frombuffer(ref, dt).astype(np.int64)
+            diff = np.abs(a - b)
+            print(f"{name}: {np.count_nonzero(diff)} pixels differ, max {diff.max()}")
+            assert diff.max() <= tol
+            return
         a, b = np.frombuffer(data, np.uint8), np.frombuffer(ref, np.uint8)
         bad = np.nonzero(a != b)[0]
         detail = f": {bad.size} bytes differ, first at byte {bad[0] if bad.size else -1}"
