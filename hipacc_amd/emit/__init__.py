@@ -55,7 +55,10 @@ class KernelClass:
     params: List[str] = field(default_factory=list)                  # constructor parameters
     param_field: Dict[str, str] = field(default_factory=dict)         # parameter -> member
     op: object = None
-    global_reduce: Optional[str] = None
+    global_reduce: Optional[str] = None   # SUM MIN MAX PROD, or USER
+    bin_type: str = ""                    # Kernel<T, TB>: TB
+    reduce_text: str = ""                 # reduce(l, r) as written
+    binning_text: str = ""                # binning(x, y, pixel) as written
     extent: Tuple[int, int] = (0, 0)
 
 
@@ -153,7 +156,13 @@ class Emitter:
         name, targ = _tpl(base[0].type)
         if name != "Kernel":
             return None
-        k = KernelClass(c.spelling, targ.split(",")[0].strip(), extent=c.extent())
+        targs = [t.strip() for t in targ.split(",")]
+        k = KernelClass(c.spelling, targs[0], extent=c.extent())
+        k.bin_type = targs[1] if len(targs) > 1 else ""
+
+        def method_text(m: C.Cursor) -> str:
+            b0, b1 = m.extent()
+            return self.src[b0:b1].decode()
         kernel_body = reduce_body = None
         kernel_text = "" 
         reduce_params: Tuple[str, str] = ("", "")
@@ -180,21 +189,24 @@ class Emitter:
                 body = [s for s in ch.children() if s.kind == "CompoundStmt"]
                 if len(ps) == 2 and body:
                     reduce_body, reduce_params = body[0].tokens(), (ps[0], ps[1])
+                    k.reduce_text = method_text(ch)
+            elif ch.kind == "CXXMethod" and ch.spelling == "binning":
+                k.binning_text = method_text(ch)
         if kernel_body is None:
             raise EmitError(f"kernel class {k.name}: no kernel() body")
         acc = [n for n, (kd, _) in k.fields.items() if kd == "Accessor"]
         msk = [n for n, (kd, _) in k.fields.items() if kd == "Mask"]
         dom = [n for n, (kd, _) in k.fields.items() if kd == "Domain"]
-        try:
-            if reduce_body is not None:
+        if reduce_body is not None and not k.binning_text:
+            try:
                 k.global_reduce = classify_reduce(reduce_body, reduce_params)
-        except Unsupported as e:
-            raise EmitError(f"kernel class {k.name}: {e}") from None
+            except Unsupported:
+                k.global_reduce = "USER"
+        if k.binning_text and not k.reduce_text:
+            raise EmitError(f"kernel class {k.name}: binning() without reduce()")
         try:
             k.op = classify_kernel(kernel_body, acc, msk, dom)
         except Unsupported as e:
-            if reduce_body is not None:
-                raise EmitError(f"kernel class {k.name}: {e}") from None
             k.op = UserOp(kernel_text, str(e))
         return k
 
@@ -228,8 +240,14 @@ class Emitter:
             text = f"// Kernel class {k.name}: {_describe(k)}"
             if isinstance(k.op, UserOp):
                 text += "\n" + self._user_struct(k)
+            if k.binning_text:
+                text += (f"\nstruct {k.name}_bins : hipacc_dev::Binning<{k.bin_type}> {{\n"
+                         f"    __device__ {k.binning_text}\n    __device__ {k.reduce_text}\n}};")
+            elif k.global_reduce == "USER":
+                text += (f"\nstruct {k.name}_reduce {{\n    __device__ {k.reduce_text}\n}};")
             self.edits.append((b, e, text))
-        self.user = any(isinstance(k.op, UserOp) for k in self.kernels.values())
+        self.user = any(isinstance(k.op, UserOp) or k.binning_text or k.global_reduce == "USER"
+                        for k in self.kernels.values())
         body = [ch for ch in main.children() if ch.kind == "CompoundStmt"][0]
         self.images: Dict[str, str] = {}
         self.bcs: Dict[str, Tuple[str, str, str]] = {}
@@ -360,13 +378,6 @@ class Emitter:
     def _launch(self, inst: str) -> str:
         k, bind, it = self._binding(inst)
         op = k.op
-        if k.global_reduce:
-            acc = bind[op.inputs[0]] if isinstance(op, PointOp) else None
-            if not isinstance(op, PointOp) or op.op != "COPY" or acc is None:
-                raise EmitError(f"{inst}: a global reduction's kernel() must copy its accessor")
-            return (f"hipaccLaunchPoint(HIPACC_POINT_COPY, {acc}, {it});\n"
-                    f"    {k.out_type} {inst}_reduced = hipaccApplyReduction<{k.out_type}>"
-                    f"({acc}, HipaccReduce::{k.global_reduce});")
         if isinstance(op, UserOp):
             return f"hipaccLaunchUser({inst}, {it});"
         if isinstance(op, PointOp):
@@ -456,7 +467,8 @@ class Emitter:
     def _chain_input(self, inst: str) -> Optional[str]:
         """The accessor a fusible kernel reads its chain value through."""
         k, bind, _ = self._binding(inst)
-        if k.global_reduce or isinstance(k.op, UserOp) or k.out_type not in self._FUSED_TYPES:
+        if (k.global_reduce or k.binning_text or isinstance(k.op, UserOp) or
+                k.out_type not in self._FUSED_TYPES):
             return None
         if isinstance(k.op, LocalOp):
             return bind[k.op.accessor]
@@ -534,12 +546,31 @@ class Emitter:
         """img.data() and k.reduced_data() anywhere in main()."""
         b, e = main.extent()
         text = self.src[b:e].decode()
+        for m in re.finditer(r"\b(\w+)\s*\.\s*binned_data\s*\(([^()]*)\)", text):
+            name = m.group(1)
+            if name not in self.instances:
+                continue
+            k, _, it = self._binding(name)
+            if not k.binning_text:
+                raise EmitError(f"{name}.binned_data(): {k.name} has no binning()")
+            rep = (f"hipaccBinUser<{k.name}_bins, {k.bin_type}>({k.name}_bins{{}}, {it}, "
+                   f"{m.group(2).strip()})")
+            s0 = b + len(text[:m.start()].encode())
+            s1 = b + len(text[:m.end()].encode())
+            if not any(x0 <= s0 < x1 for x0, x1, _ in self.edits):
+                self.edits.append((s0, s1, rep))
         for m in re.finditer(r"\b(\w+)\s*\.\s*(data|reduced_data)\s*\(\s*\)", text):
             name, what = m.group(1), m.group(2)
             if what == "data" and name in self.images:
                 rep = f"hipaccReadMemory<{self.images[name]}>({name})"
             elif what == "reduced_data" and name in self.instances:
-                rep = f"{name}_reduced"
+                k, _, it = self._binding(name)
+                if k.global_reduce == "USER":
+                    rep = f"hipaccReduceUser({k.name}_reduce{{}}, {it})"
+                elif k.global_reduce:
+                    rep = f"hipaccApplyReduction<{k.out_type}>({it}, HipaccReduce::{k.global_reduce})"
+                else:
+                    raise EmitError(f"{name}.reduced_data(): {k.name} has no reduce()")
             else:
                 continue
             s0 = b + len(text[:m.start()].encode())
@@ -571,10 +602,21 @@ class Emitter:
 
 def _describe(k: KernelClass) -> str:
     op = k.op
+    if k.binning_text:
+        extra = "; binning() and reduce() as device code (hipaccBinUser)"
+    elif k.global_reduce == "USER":
+        extra = "; reduce() as device code (hipaccReduceUser)"
+    elif k.global_reduce:
+        extra = f"; global {k.global_reduce} reduction (hipaccApplyReduction)"
+    else:
+        extra = ""
+    return _describe_kernel(k) + extra
+
+
+def _describe_kernel(k: KernelClass) -> str:
+    op = k.op
     if isinstance(op, UserOp):
         return "device code, the kernel() body compiled as written (hipaccLaunchUser)"
-    if k.global_reduce:
-        return f"global {k.global_reduce} reduction (hipaccApplyReduction)"
     if isinstance(op, PointOp):
         return f"point operator {op.op} (hipaccLaunchPoint)"
     return (f"local operator {op.reduce} over {'mask ' + op.mask if op.mask else 'domain ' + op.domain}"

@@ -21,6 +21,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <memory>
 #include <vector>
 
@@ -168,6 +169,102 @@ __global__ __launch_bounds__(256) void user_kernel(F f, TO* out, int stride, int
   out[(long long)(oy + y) * stride + ox + x] = k.output();
 }
 
+// The binning side of a Kernel<T, TB> class: binning(x, y, pixel) sets
+// bin(idx) = value, reduce(l, r) combines values (dsl/kernel.hpp:163-209).
+template <typename TB> struct Binning {
+  unsigned bidx_ = 0xffffffffu;
+  TB bval_{};
+  unsigned nbins_ = 0;
+  __device__ __forceinline__ TB& bin(unsigned idx) {
+    bidx_ = idx;
+    return bval_;
+  }
+  __device__ __forceinline__ unsigned num_bins() const { return nbins_; }
+};
+
+// Combines v into *p with the user's reduce(), as one atomic update of the
+// 32-bit word (compare-and-swap until no other thread changed it between).
+template <typename F, typename TB>
+__device__ __forceinline__ void combine_at(const F& k, TB* p, TB v) {
+  static_assert(sizeof(TB) == 4, "bins of 32-bit values");
+  unsigned* w = reinterpret_cast<unsigned*>(p);
+  unsigned old = *w, seen;
+  do {
+    seen = old;
+    const TB nv = k.reduce(__builtin_bit_cast(TB, seen), v);
+    old = atomicCAS(w, seen, __builtin_bit_cast(unsigned, nv));
+  } while (old != seen);
+}
+
+// Bins of the output image: each block combines its pixels' values into
+// bins in LDS, then its bins into the global bins (zero before the launch,
+// as the DSL's binned_data() starts from TB{}).  reduce() must be
+// associative and commutative, as any parallel binning assumes.
+template <typename F, typename T, typename TB>
+__global__ __launch_bounds__(256) void user_binning_kernel(F f, const T* img, int stride, int ox,
+                                                           int oy, int w, int h, TB* bins,
+                                                           unsigned nbins) {
+  extern __shared__ __align__(16) unsigned char user_bins_lds[];
+  TB* lb = reinterpret_cast<TB*>(user_bins_lds);
+  for (unsigned b = threadIdx.x; b < nbins; b += 256) lb[b] = TB{};
+  __syncthreads();
+  F k = f;
+  k.nbins_ = nbins;
+  const long long n = (long long)w * h;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (long long)gridDim.x * 256) {
+    const int y = (int)(i / w), x = (int)(i - (long long)y * w);
+    k.bidx_ = 0xffffffffu;
+    k.binning((unsigned)x, (unsigned)y, img[(long long)(oy + y) * stride + ox + x]);
+    if (k.bidx_ < nbins) combine_at(k, lb + k.bidx_, k.bval_);
+  }
+  __syncthreads();
+  for (unsigned b = threadIdx.x; b < nbins; b += 256)
+    if (lb[b] != TB{}) combine_at(k, bins + b, lb[b]);
+}
+
+// A user reduction reduce(l, r) over an image window: per-block trees in
+// LDS, then one block folds the per-block results, both in a fixed order.
+template <typename F, typename T>
+__global__ __launch_bounds__(256) void user_reduce_kernel(F f, const T* img, int stride, int ox,
+                                                          int oy, int w, int h, T* partial) {
+  __shared__ T red[256];
+  const F k = f;
+  const long long n = (long long)w * h;
+  const long long per = (n + gridDim.x - 1) / gridDim.x;
+  const long long i0 = per * blockIdx.x, i1 = i0 + per < n ? i0 + per : n;
+  bool have = false;
+  T acc{};
+  for (long long i = i0 + threadIdx.x; i < i1; i += 256) {
+    const int y = (int)(i / w), x = (int)(i - (long long)y * w);
+    const T v = img[(long long)(oy + y) * stride + ox + x];
+    acc = have ? k.reduce(acc, v) : v;
+    have = true;
+  }
+  __shared__ int valid[256];
+  red[threadIdx.x] = acc;
+  valid[threadIdx.x] = have;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s && valid[threadIdx.x + s]) {
+      red[threadIdx.x] = valid[threadIdx.x] ? k.reduce(red[threadIdx.x], red[threadIdx.x + s])
+                                            : red[threadIdx.x + s];
+      valid[threadIdx.x] = 1;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+template <typename F, typename T>
+__global__ __launch_bounds__(64) void user_reduce_final(F f, T* partial, int nparts) {
+  if (threadIdx.x != 0) return;
+  const F k = f;
+  T acc = partial[0];
+  for (int i = 1; i < nparts; ++i) acc = k.reduce(acc, partial[i]);
+  partial[0] = acc;
+}
+
 }  // namespace hipacc_dev
 
 // ---- host side ---------------------------------------------------------------
@@ -235,6 +332,65 @@ int hipaccLaunchUser(const F& f, const HipaccAccessor<TO>& is,
   const int st = hipaccCheck(e == hipSuccess ? HIPACC_OK : HIPACC_ERR_HIP, "hipaccLaunchUser");
   if (ep) ep->post_kernel();
   return st;
+}
+
+// binned_data(num_bins) of a Kernel<T, TB> class over the image its
+// iteration space wrote: a new TB[num_bins] the caller owns.  Blocking.
+template <typename F, typename TB, typename T>
+TB* hipaccBinUser(const F& f, const HipaccAccessor<T>& is, unsigned num_bins,
+                  const HipaccExecutionParameterHip& ep = nullptr) {
+  TB* result = new TB[num_bins]();
+  const hipacc_view v = hipaccView(is);
+  void* bins = nullptr;
+  int32_t stride = 0;
+  if (hipaccCheck(hipacc_hip_alloc2d(&bins, &stride, (int32_t)num_bins, 1, 4, 0), "hipaccBinUser"))
+    return result;
+  hipStream_t s = static_cast<hipStream_t>(hipaccStreamOf(ep));
+  hipaccCheck(hipacc_hip_fill(bins, 0u, num_bins, 4, s), "hipaccBinUser");
+  const long long n = (long long)v.width * v.height;
+  const int blocks = (int)std::min<long long>(std::max<long long>((n + 4095) / 4096, 1), 1024);
+  if (ep) ep->pre_kernel();
+  hipLaunchKernelGGL((hipacc_dev::user_binning_kernel<F, T, TB>), dim3(blocks), dim3(256),
+                     (size_t)num_bins * sizeof(TB), s, f, static_cast<const T*>(v.data), v.stride,
+                     v.offset_x, v.offset_y, v.width, v.height, static_cast<TB*>(bins), num_bins);
+  const hipError_t e = hipGetLastError();
+  if (!hipaccCheck(e == hipSuccess ? HIPACC_OK : HIPACC_ERR_HIP, "hipaccBinUser"))
+    hipaccCheck(hipacc_hip_download2d(result, bins, (int32_t)num_bins, (int32_t)num_bins, 1, 4, s),
+                "hipaccBinUser");
+  if (ep) ep->post_kernel();
+  (void)hipacc_hip_free(bins);
+  return result;
+}
+
+// reduced_data() of a Kernel class with a reduce(l, r) the library has no
+// operator for, over the image its iteration space wrote.  Blocking.
+template <typename F, typename T>
+T hipaccReduceUser(const F& f, const HipaccAccessor<T>& is,
+                   const HipaccExecutionParameterHip& ep = nullptr) {
+  T result{};
+  const hipacc_view v = hipaccView(is);
+  constexpr int kBlocks = 256;
+  void* part = nullptr;
+  int32_t stride = 0;
+  if (hipaccCheck(hipacc_hip_alloc2d(&part, &stride, kBlocks, 1, (int32_t)sizeof(T), 0),
+                  "hipaccReduceUser"))
+    return result;
+  hipStream_t s = static_cast<hipStream_t>(hipaccStreamOf(ep));
+  const long long n = (long long)v.width * v.height;
+  const int blocks = (int)std::min<long long>(std::max<long long>((n + 255) / 256, 1), kBlocks);
+  if (ep) ep->pre_kernel();
+  hipLaunchKernelGGL((hipacc_dev::user_reduce_kernel<F, T>), dim3(blocks), dim3(256), 0, s, f,
+                     static_cast<const T*>(v.data), v.stride, v.offset_x, v.offset_y, v.width,
+                     v.height, static_cast<T*>(part));
+  hipLaunchKernelGGL((hipacc_dev::user_reduce_final<F, T>), dim3(1), dim3(64), 0, s, f,
+                     static_cast<T*>(part), blocks);
+  const hipError_t e = hipGetLastError();
+  if (!hipaccCheck(e == hipSuccess ? HIPACC_OK : HIPACC_ERR_HIP, "hipaccReduceUser"))
+    hipaccCheck(hipacc_hip_download2d(&result, part, 1, 1, 1, (int32_t)sizeof(T), s),
+                "hipaccReduceUser");
+  if (ep) ep->post_kernel();
+  (void)hipacc_hip_free(part);
+  return result;
 }
 
 #endif  // HIPACC_HIP_DEVICE_HPP

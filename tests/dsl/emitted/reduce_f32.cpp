@@ -19,11 +19,11 @@ namespace hipacc { namespace math { using std::min; using std::max; } }
 using namespace hipacc;
 using namespace hipacc::math;
 
-// Kernel class MinReduction: global MIN reduction (hipaccApplyReduction)
+// Kernel class MinReduction: point operator COPY (hipaccLaunchPoint); global MIN reduction (hipaccApplyReduction)
 
-// Kernel class MaxReduction: global MAX reduction (hipaccApplyReduction)
+// Kernel class MaxReduction: point operator COPY (hipaccLaunchPoint); global MAX reduction (hipaccApplyReduction)
 
-// Kernel class SumReduction: global SUM reduction (hipaccApplyReduction)
+// Kernel class SumReduction: point operator COPY (hipaccLaunchPoint); global SUM reduction (hipaccApplyReduction)
 
 static float pixel_value(int x, int y) {
     uint32_t h = (uint32_t)x * 2246822519u + (uint32_t)y * 3266489917u;
@@ -47,16 +47,13 @@ HIPACC_CODEGEN int main(int argc, const char **argv) {
 
     // MinReduction rmin(iter, acc): launched at rmin.execute()
     hipaccLaunchPoint(HIPACC_POINT_COPY, acc, iter);
-    float rmin_reduced = hipaccApplyReduction<float>(acc, HipaccReduce::MIN);
-    float vmin = rmin_reduced;
+    float vmin = hipaccApplyReduction<float>(iter, HipaccReduce::MIN);
     // MaxReduction rmax(iter, acc): launched at rmax.execute()
     hipaccLaunchPoint(HIPACC_POINT_COPY, acc, iter);
-    float rmax_reduced = hipaccApplyReduction<float>(acc, HipaccReduce::MAX);
-    float vmax = rmax_reduced;
+    float vmax = hipaccApplyReduction<float>(iter, HipaccReduce::MAX);
     // SumReduction rsum(iter, acc): launched at rsum.execute()
     hipaccLaunchPoint(HIPACC_POINT_COPY, acc, iter);
-    float rsum_reduced = hipaccApplyReduction<float>(acc, HipaccReduce::SUM);
-    float vsum = rsum_reduced;
+    float vsum = hipaccApplyReduction<float>(iter, HipaccReduce::SUM);
 
     // the sum in double and the sum of magnitudes, for the error bound of
     // the float sums
