@@ -177,6 +177,8 @@ SIGNATURES = {
     "hipacc_hip_fill_random": (c_int32, [POINTER(View), c_uint64, c_void_p]),
     "hipacc_hip_set_timing": (c_int32, [c_int32]),
     "hipacc_hip_last_kernel_timing": (c_float, []),
+    "hipacc_hip_get_timing": (c_int32, []),
+    "hipacc_hip_set_last_kernel_timing": (c_int32, [c_float]),
     "hipacc_hip_local_variant": (c_int32, [POINTER(LocalDesc), c_int32, c_int32, c_void_p]),
     "hipacc_hip_separable_variant": (c_int32, [POINTER(SeparableDesc), c_int32, c_int32, c_void_p]),
     "hipacc_hip_tune_local": (c_int32, [POINTER(LocalDesc), c_int32, POINTER(TuneEntry), c_void_p]),

@@ -354,6 +354,12 @@ int hipacc_hip_set_timing(int enabled) {
   return HIPACC_OK;
 }
 float hipacc_hip_last_kernel_timing(void) { return g_last_ms; }
+int hipacc_hip_get_timing(void) { return g_timing ? 1 : 0; }
+int hipacc_hip_set_last_kernel_timing(float ms) {
+  if (!(ms >= 0.0f)) return set_error(HIPACC_ERR_INVALID_ARG, "kernel timing %f ms", (double)ms);
+  g_last_ms = ms;
+  return HIPACC_OK;
+}
 
 }  // extern "C"
 
@@ -1436,7 +1442,8 @@ int hipacc_hip_resample(const hipacc_resample_desc* d, void* stream) {
 int hipacc_hip_alloc2d(void** ptr, int32_t* stride_elems, int32_t width, int32_t height,
                        int32_t elem_bytes, int32_t alignment) {
   if (!ptr || !stride_elems || width <= 0 || height <= 0 ||
-      !(elem_bytes == 1 || elem_bytes == 2 || elem_bytes == 4))
+      !(elem_bytes == 1 || elem_bytes == 2 || elem_bytes == 4 || elem_bytes == 8 ||
+        elem_bytes == 16))
     return set_error(HIPACC_ERR_INVALID_ARG, "alloc2d: bad arguments");
   if (alignment <= 0) alignment = 256;
   if (alignment % elem_bytes) return set_error(HIPACC_ERR_INVALID_ARG, "alloc2d: alignment");

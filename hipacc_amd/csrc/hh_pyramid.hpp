@@ -73,6 +73,76 @@ HH_HD float interp_w_l3(float d) {
   return 0.0f;
 }
 
+// The interpolated value at mapped position (mx, my) (dsl/image.hpp:
+// 390-532); read(x, y) returns the pixel at absolute image position (x, y)
+// through the accessor's boundary handling, as a float.  NN reads the pixel
+// at the truncated position; LF is the bilinear combination; B5 / CF / L3
+// sum rows of NX taps (left to right, one rounding per product and add)
+// weighted per row:
+//   B5: frac + 0.5, taps x_int + k, weight b5(frac - k);
+//   CF: taps x_int - 1 + k, weight cf((frac - 1) + k);
+//   L3: taps x_int - 2 + k, rows y_int - 1 + k, weight l3((frac - 2) + k),
+//       and in the second row the fifth tap takes the sixth tap's weight.
+template <int MODE, typename R>
+HH_HD float interp_sample(const R& read, float mx, float my) {
+  if constexpr (MODE == HIPACC_INTERP_NN) {
+    return read((int)mx, (int)my);
+  } else if constexpr (MODE == HIPACC_INTERP_LF) {
+    int xi, yi;
+    float fx, fy;
+    interp_lf(mx, xi, fx);
+    interp_lf(my, yi, fy);
+    return interp_bilinear(fx, fy, read(xi, yi), read(xi + 1, yi), read(xi, yi + 1),
+                           read(xi + 1, yi + 1));
+  } else {
+    int xi, yi;
+    float fx, fy;
+    interp_lf(mx, xi, fx);
+    interp_lf(my, yi, fy);
+    constexpr int NX = MODE == HIPACC_INTERP_L3 ? 6 : 4;
+    constexpr int NY = MODE == HIPACC_INTERP_L3 ? 6 : 4;
+    float bx, by;
+    int x0, y0;
+    if constexpr (MODE == HIPACC_INTERP_B5) {
+      bx = fx + 0.5f;
+      by = fy + 0.5f;
+      x0 = xi;
+      y0 = yi;
+    } else if constexpr (MODE == HIPACC_INTERP_CF) {
+      bx = fx - 1.0f;
+      by = fy - 1.0f;
+      x0 = xi - 1;
+      y0 = yi - 1;
+    } else {
+      bx = fx - 2.0f;
+      by = fy - 2.0f;
+      x0 = xi - 2;
+      y0 = yi - 1;
+    }
+    float v = 0.0f;
+#pragma unroll
+    for (int r = 0; r < NY; ++r) {
+      float row = 0.0f;
+#pragma unroll
+      for (int k = 0; k < NX; ++k) {
+        float w;
+        if constexpr (MODE == HIPACC_INTERP_B5) w = interp_w_b5(bx - (float)k);
+        else if constexpr (MODE == HIPACC_INTERP_CF) w = interp_w_cf(bx + (float)k);
+        else w = interp_w_l3(bx + (float)((r == 1 && k == 4) ? 5 : k));
+        const float t = read(x0 + k, y0 + r) * w;
+        row = k == 0 ? t : row + t;
+      }
+      float wy;
+      if constexpr (MODE == HIPACC_INTERP_B5) wy = interp_w_b5(by - (float)r);
+      else if constexpr (MODE == HIPACC_INTERP_CF) wy = interp_w_cf(by + (float)r);
+      else wy = interp_w_l3(by + (float)r);
+      const float t = row * wy;
+      v = r == 0 ? t : v + t;
+    }
+    return v;
+  }
+}
+
 struct ResampleArgs {
   const void* in;              // in_dtype pixels (F32 or U8)
   const float* in2;

@@ -26,80 +26,13 @@ __device__ __forceinline__ float rs_read(const ResampleArgs& a, int x, int y) {
   return (float)static_cast<const T*>(a.in)[(long long)y * a.in_stride + x];
 }
 
-// Sum of NX taps of source row yr starting at column x0, weights w(fx + k)
-// (k = 0 .. NX-1, with the L3 quirk of the second row: its fifth tap takes
-// the sixth tap's weight), left to right with one rounding per product and
-// per add.
-template <typename T, int MODE>
-__device__ __forceinline__ float rs_row(const ResampleArgs& a, int x0, int yr, float fx, int row) {
-  constexpr int NX = MODE == HIPACC_INTERP_L3 ? 6 : 4;
-  float acc = 0.0f;
-#pragma unroll
-  for (int k = 0; k < NX; ++k) {
-    float w;
-    if constexpr (MODE == HIPACC_INTERP_B5) w = interp_w_b5(fx - (float)k);
-    else if constexpr (MODE == HIPACC_INTERP_CF) w = interp_w_cf(fx + (float)k);
-    else w = interp_w_l3(fx + (float)((row == 1 && k == 4) ? 5 : k));
-    const float t = rs_read<T>(a, x0 + k, yr) * w;
-    acc = k == 0 ? t : acc + t;
-  }
-  return acc;
-}
-
 // Interpolation::interpolate (dsl/image.hpp:390-532) at iteration-space
 // pixel (x, y) read at (x + xf, y + yf).
 template <typename T, int MODE>
 __device__ __forceinline__ float rs_value(const ResampleArgs& a, int x, int y) {
   const float mx = interp_map(a.sx, x + a.xf, a.lo_x);
   const float my = interp_map(a.sy, y + a.yf, a.lo_y);
-  if constexpr (MODE == HIPACC_INTERP_NN) {
-    return rs_read<T>(a, (int)mx, (int)my);
-  } else if constexpr (MODE == HIPACC_INTERP_LF) {
-    int xi, yi;
-    float fx, fy;
-    interp_lf(mx, xi, fx);
-    interp_lf(my, yi, fy);
-    return interp_bilinear(fx, fy, rs_read<T>(a, xi, yi), rs_read<T>(a, xi + 1, yi),
-                           rs_read<T>(a, xi, yi + 1), rs_read<T>(a, xi + 1, yi + 1));
-  } else {
-    int xi, yi;
-    float fx, fy;
-    interp_lf(mx, xi, fx);
-    interp_lf(my, yi, fy);
-    // B5: frac + 0.5, taps x_int + k, weight b5(frac - k);
-    // CF: taps x_int - 1 + k, weight cf((frac - 1) + k);
-    // L3: taps x_int - 2 + k, rows y_int - 1 + k, weight l3((frac - 2) + k)
-    constexpr int NY = MODE == HIPACC_INTERP_L3 ? 6 : 4;
-    float bx, by;
-    int x0, y0;
-    if constexpr (MODE == HIPACC_INTERP_B5) {
-      bx = fx + 0.5f;
-      by = fy + 0.5f;
-      x0 = xi;
-      y0 = yi;
-    } else if constexpr (MODE == HIPACC_INTERP_CF) {
-      bx = fx - 1.0f;
-      by = fy - 1.0f;
-      x0 = xi - 1;
-      y0 = yi - 1;
-    } else {
-      bx = fx - 2.0f;
-      by = fy - 2.0f;
-      x0 = xi - 2;
-      y0 = yi - 1;
-    }
-    float v = 0.0f;
-#pragma unroll
-    for (int r = 0; r < NY; ++r) {
-      float wy;
-      if constexpr (MODE == HIPACC_INTERP_B5) wy = interp_w_b5(by - (float)r);
-      else if constexpr (MODE == HIPACC_INTERP_CF) wy = interp_w_cf(by + (float)r);
-      else wy = interp_w_l3(by + (float)r);
-      const float t = rs_row<T, MODE>(a, x0, y0 + r, bx, r) * wy;
-      v = r == 0 ? t : v + t;
-    }
-    return v;
-  }
+  return interp_sample<MODE>([&](int px, int py) { return rs_read<T>(a, px, py); }, mx, my);
 }
 
 template <typename T, int MODE>

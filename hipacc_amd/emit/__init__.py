@@ -59,7 +59,22 @@ class KernelClass:
     bin_type: str = ""                    # Kernel<T, TB>: TB
     reduce_text: str = ""                 # reduce(l, r) as written
     binning_text: str = ""                # binning(x, y, pixel) as written
+    body_text: str = ""                   # kernel() body as written
     extent: Tuple[int, int] = (0, 0)
+
+
+@dataclass
+class AccInfo:
+    """An Accessor / IterationSpace of main(): the image expression it reads
+    (an image, or a pyramid level `p(i)`), its boundary handling, its
+    interpolation (None, or NN / LF / B5 / CF / L3), whether it crops, and
+    the name it was constructed from (image or boundary condition)."""
+    img: str
+    mode: str
+    value: str
+    interp: Optional[str]
+    crop: bool
+    src: str
 
 
 _TPL = re.compile(r"^(?:const\s+)?(?:hipacc::)?(\w+)\s*(?:<\s*(.+?)\s*>)?\s*&?$")
@@ -137,16 +152,41 @@ class Emitter:
         if rd:
             args += ["-resource-dir", rd]
         args += [f"-I{d}" for d in include_dirs] + [f"-D{d}" for d in defines]
-        self.tu = C.TranslationUnit(path, args)
+        self.tu = C.TranslationUnit(path, args, C.DETAILED_PREPROCESSING)
         with open(path, "rb") as f:
             self.src = f.read()
         main_errors = [e for e in self.tu.errors if e.startswith(path)]
         if main_errors:
             raise EmitError("the program does not parse:\n" + "\n".join(main_errors))
+        self.macros = self._object_macros()
         self.kernels: Dict[str, KernelClass] = {}
         self.field_types: Dict[Tuple[str, str], str] = {}
         self.var_types: Dict[str, str] = {}
         self.edits: List[Tuple[int, int, str]] = []
+
+    def _object_macros(self) -> Dict[str, List[str]]:
+        """The program's object-like macros (name -> replacement tokens), for
+        kernel bodies written with them (`TYPE pixel = in();`)."""
+        out: Dict[str, List[str]] = {}
+        for c in self.tu.cursor.children():
+            if c.kind != "macro definition" or not c.in_main_file():
+                continue
+            b, _ = c.extent()
+            name = c.spelling
+            if self.src[b + len(name):b + len(name) + 1] == b"(":
+                continue                          # function-like
+            out[name] = c.tokens()[1:]
+        return out
+
+    def _expand(self, toks: Sequence[str], depth: int = 0) -> List[str]:
+        out: List[str] = []
+        for t in toks:
+            rep = self.macros.get(t)
+            if rep is None or depth >= 8:
+                out.append(t)
+            else:
+                out.extend(self._expand(rep, depth + 1))
+        return out
 
     # ---- kernel classes ---------------------------------------------------------
     def _kernel_class(self, c: C.Cursor) -> Optional[KernelClass]:
@@ -177,7 +217,7 @@ class Emitter:
                 k.param_field.update(_member_inits(ch.tokens(), k.params))
             elif ch.kind == "CXXMethod" and ch.spelling == "kernel":
                 body = [s for s in ch.children() if s.kind == "CompoundStmt"]
-                kernel_body = body[0].tokens() if body else None
+                kernel_body = self._expand(body[0].tokens()) if body else None
                 if body:
                     b0, b1 = body[0].extent()
                     kernel_text = self.src[b0:b1].decode()
@@ -188,12 +228,13 @@ class Emitter:
                 ps = [p.spelling for p in ch.children() if p.kind == "ParmDecl"]
                 body = [s for s in ch.children() if s.kind == "CompoundStmt"]
                 if len(ps) == 2 and body:
-                    reduce_body, reduce_params = body[0].tokens(), (ps[0], ps[1])
+                    reduce_body, reduce_params = self._expand(body[0].tokens()), (ps[0], ps[1])
                     k.reduce_text = method_text(ch)
             elif ch.kind == "CXXMethod" and ch.spelling == "binning":
                 k.binning_text = method_text(ch)
         if kernel_body is None:
             raise EmitError(f"kernel class {k.name}: no kernel() body")
+        k.body_text = kernel_text
         acc = [n for n, (kd, _) in k.fields.items() if kd == "Accessor"]
         msk = [n for n, (kd, _) in k.fields.items() if kd == "Mask"]
         dom = [n for n, (kd, _) in k.fields.items() if kd == "Domain"]
@@ -233,6 +274,21 @@ class Emitter:
         if main is None:
             raise EmitError("no main() definition")
         self._collect_var_types(main)
+        body = [ch for ch in main.children() if ch.kind == "CompoundStmt"][0]
+        mb, me = main.extent()
+        self.main_text = self.src[mb:me].decode()
+        # A kernel the library would run as an operator but whose instance
+        # reads through an interpolating accessor is compiled as device code
+        # instead; main() is then walked again.
+        for _ in range(2):
+            self._walk_main(body)
+            forced = {self.instances[i][0] for i in self.interp_insts}
+            forced = {n for n in forced if not isinstance(self.kernels[n].op, UserOp)}
+            if not forced:
+                break
+            for n in forced:
+                k = self.kernels[n]
+                k.op = UserOp(k.body_text, "reads through an interpolating accessor")
         for k in self.kernels.values():
             b, e = k.extent
             if self.src[e:e + 1] == b";":
@@ -248,21 +304,6 @@ class Emitter:
             self.edits.append((b, e, text))
         self.user = any(isinstance(k.op, UserOp) or k.binning_text or k.global_reduce == "USER"
                         for k in self.kernels.values())
-        body = [ch for ch in main.children() if ch.kind == "CompoundStmt"][0]
-        self.images: Dict[str, str] = {}
-        self.bcs: Dict[str, Tuple[str, str, str]] = {}
-        self.accessors: Dict[str, Tuple[str, str, Optional[str]]] = {}
-        self.instances: Dict[str, Tuple[str, List[str]]] = {}
-        self.masks: Dict[str, str] = {}
-        self.timing = b"hipacc_last_kernel_timing" in self.src
-        self.launches: List[Tuple[str, int, int]] = []
-        self.image_args: Dict[str, List[str]] = {}
-        self.iter_crop: Dict[str, bool] = {}
-        self.acc_src: Dict[str, str] = {}
-        for st in body.children():
-            self._statement(st)
-        mb, me = main.extent()
-        self.main_text = self.src[mb:me].decode()
         for chain in self._chains():
             if len(chain) == 1:
                 inst, b, e = chain[0]
@@ -279,27 +320,98 @@ class Emitter:
         self.edits.append((b0 + 1, b0 + 1, init))
         return self._apply()
 
+    def _walk_main(self, body: C.Cursor) -> None:
+        self.edits = []
+        self.images: Dict[str, str] = {}
+        self.pyramids: Dict[str, str] = {}
+        self.bcs: Dict[str, Tuple[str, str, str]] = {}
+        self.accessors: Dict[str, AccInfo] = {}
+        self.instances: Dict[str, Tuple[str, List[str]]] = {}
+        self.inst_accs: Dict[str, Dict[str, AccInfo]] = {}
+        self.interp_insts: List[str] = []
+        self.masks: Dict[str, str] = {}
+        self.domains: set = set()
+        self.timing = b"hipacc_last_kernel_timing" in self.src
+        self.launches: List[Tuple[str, int, int]] = []
+        self.launch_scope: Dict[int, Tuple[int, int]] = {}
+        self.image_args: Dict[str, List[str]] = {}
+        self.traversor = False
+        for st in body.children():
+            self._visit(st, body.extent())
+
+    def _visit(self, st: C.Cursor, scope: Tuple[int, int]) -> None:
+        """A statement of main() (or of a block, loop or lambda inside it)."""
+        if st.kind == "DeclStmt":
+            self._statement(st)
+            return
+        if st.kind == "CallExpr" and self._call(st, scope):
+            return
+        for ch in st.children():
+            self._visit(ch, ch.extent() if ch.kind == "CompoundStmt" else scope)
+
     def _statement(self, st: C.Cursor) -> None:
         toks = st.tokens()
         b, e = st.extent()
         if self.src[e:e + 1] == b";":
             e += 1
-        if st.kind == "DeclStmt":
-            decls = [ch for ch in st.children() if ch.kind == "VarDecl"]
-            if len(decls) != 1:
-                return
-            v = decls[0]
-            name, ty = v.spelling, v.type
-            kind, targ = _tpl(ty)
-            i = toks.index(name) if name in toks else -1
-            args = _args(toks, i + 1)[0] if 0 <= i < len(toks) - 1 and toks[i + 1] == "(" else []
-            rep = self._declaration(name, kind, targ, args)
-            if rep is not None:
-                self.edits.append((b, e, rep))
-        elif st.kind == "CallExpr":
-            m = re.match(r"^(\w+)\.execute\(\)$", "".join(toks).rstrip(";"))
-            if m and m.group(1) in self.instances:
-                self.launches.append((m.group(1), b, e))
+        decls = [ch for ch in st.children() if ch.kind == "VarDecl"]
+        if len(decls) != 1:
+            return
+        v = decls[0]
+        name, ty = v.spelling, v.type
+        kind, targ = _tpl(ty)
+        i = toks.index(name) if name in toks else -1
+        args = _args(toks, i + 1)[0] if 0 <= i < len(toks) - 1 and toks[i + 1] == "(" else []
+        rep = self._declaration(name, kind, targ, args)
+        if rep is not None:
+            self.edits.append((b, e, rep))
+
+    def _call(self, st: C.Cursor, scope: Tuple[int, int]) -> bool:
+        """A call statement the emitter rewrites; False to look inside it."""
+        toks = st.tokens()
+        text = "".join(toks).rstrip(";")
+        b, e = st.extent()
+        end = e + 1 if self.src[e:e + 1] == b";" else e
+        m = re.match(r"^(\w+)\.execute\(\)$", text)
+        if m and m.group(1) in self.instances:
+            self.launches.append((m.group(1), b, end))
+            self.launch_scope[b] = scope
+            return True
+        if toks and toks[0] == "traverse":
+            # traverse(...) -> the runtime's traversor (lib/Rewrite/Rewrite.cpp:
+            # 2083-2100 declares it before the first call)
+            rep = "hipaccTraversor.hipaccTraverse"
+            if not self.traversor:
+                rep = "HipaccPyramidTraversor hipaccTraversor;\n    " + rep
+                self.traversor = True
+            self.edits.append((b, b + len("traverse"), rep))
+            for ch in st.children():
+                self._visit(ch, scope)
+            return True
+        m = re.match(r"^(\w+)=(\w+)$", text)
+        if m and m.group(1) in self.images:
+            dst, src = m.group(1), m.group(2)
+            if src in self.images:      # Image = Image: a device copy
+                rep = f"hipaccCopyMemory({src}, {dst});"
+            else:                        # Image = host pointer: an upload
+                rep = f"hipaccWriteMemory<{self.images[dst]}>({dst}, {src});"
+            self.edits.append((b, end, rep))
+            return True
+        m = re.match(r"^(\w+)\((.+)\)=(.+)$", text)
+        if m and m.group(1) in self.masks:
+            what = "domain_at" if m.group(1) in self.domains else "coef_at"
+            self.edits.append((b, end, f"{m.group(1)}.{what}({m.group(2)}) = {m.group(3)};"))
+            return True
+        return False
+
+    def _img_type(self, expr: str) -> Optional[str]:
+        """The pixel type of an image expression: an image, or a pyramid level."""
+        if expr in self.images:
+            return self.images[expr]
+        m = re.match(r"^(\w+)\((.+)\)$", expr)
+        if m and m.group(1) in self.pyramids:
+            return self.pyramids[m.group(1)]
+        return None
 
     def _declaration(self, name: str, kind: str, targ: str, args: List[str]) -> Optional[str]:
         if kind == "Image":
@@ -309,6 +421,12 @@ class Emitter:
             if len(args) < 2:
                 raise EmitError(f"Image {name}: expected (width, height[, data])")
             return f"HipaccImageHip<{targ}> {name} = hipaccCreateMemory<{targ}>({host}, {args[0]}, {args[1]});"
+        if kind == "Pyramid":
+            if len(args) != 2 or args[0] not in self.images:
+                raise EmitError(f"Pyramid {name}: expected (image, depth)")
+            self.pyramids[name] = targ
+            return (f"HipaccPyramidHip<{targ}> {name} = "
+                    f"hipaccCreatePyramid<{targ}>({args[0]}, {args[1]});")
         if kind == "Mask":
             if len(args) != 1:
                 raise EmitError(f"Mask {name}: expected one coefficient array")
@@ -319,13 +437,21 @@ class Emitter:
             sy, sx = dims
             return f"HipaccMask<{targ}> {name}({sx}, {sy}, &{args[0]}[0][0]);"
         if kind == "Domain":
+            self.domains.add(name)
             if len(args) == 2:
                 self.masks[name] = "float"
                 return f"HipaccMask<float> {name}({args[0]}, {args[1]});"
             if len(args) == 1 and args[0] in self.masks:
                 self.masks[name] = self.masks[args[0]]
                 return f"auto {name} = {args[0]};\n    {name}.domain_from_coef();"
-            raise EmitError(f"Domain {name}: expected (size_x, size_y) or (mask)")
+            dims = re.findall(r"\[(\d+)\]", self.var_types.get(args[0], "")) if args else []
+            if len(args) == 1 and len(dims) == 2:
+                # Domain(const uchar (&)[sy][sx]): the array's nonzero entries
+                self.masks[name] = "float"
+                sy, sx = dims
+                return (f"HipaccMask<float> {name}({sx}, {sy});\n"
+                        f"    {name}.domain_write(&{args[0]}[0][0]);")
+            raise EmitError(f"Domain {name}: expected (size_x, size_y), (mask) or a 2D array")
         if kind == "BoundaryCondition":
             mode = [a for a in args if re.match(r"^(?:hipacc::)?Boundary::\w+$", a)]
             if not args or len(mode) != 1:
@@ -337,29 +463,38 @@ class Emitter:
         if kind in ("Accessor", "IterationSpace"):
             if not args:
                 raise EmitError(f"{kind} {name}: no image")
-            if any("Interpolate" in a for a in args):
-                raise EmitError(f"{kind} {name}: interpolating accessors are launched as resampling "
-                                "operators (hipacc_hip_resample), not emitted")
+            ip = [a for a in args if re.match(r"^(?:hipacc::)?Interpolate::\w+$", a)]
+            args = [a for a in args if a not in ip]
+            interp = ip[0].split("::")[-1] if ip else None
+            if interp == "NO":
+                interp = None
+            if interp and kind != "Accessor":
+                raise EmitError(f"{kind} {name}: an iteration space does not interpolate")
             src = args[0]
             if src in self.bcs:
                 img, mode, value = self.bcs[src]
             else:
                 img, mode, value = src, "UNDEFINED", "0"
-            if img not in self.images:
+            if self._img_type(img) is None:
                 raise EmitError(f"{kind} {name}: unknown image {img}")
-            self.accessors[name] = (img, mode, value)
-            self.acc_src[name] = src
-            self.iter_crop[name] = len(args) != 1
-            crop = "" if len(args) == 1 else ", " + ", ".join(args[1:5])
             if len(args) not in (1, 5):
                 raise EmitError(f"{kind} {name}: expected (image) or (image, w, h, x, y)")
+            self.accessors[name] = AccInfo(img, mode, value, interp, len(args) != 1, src)
+            crop = "" if len(args) == 1 else ", " + ", ".join(args[1:5])
             return f"HipaccAccessor<{targ}> {name} = hipaccMakeAccessor({img}{crop});"
         if kind in self.kernels:
             self.instances[name] = (kind, args)
+            self.inst_accs[name] = {a: self.accessors[a] for a in args if a in self.accessors}
+            if any(a.interp for a in self.inst_accs[name].values()):
+                self.interp_insts.append(name)
             if isinstance(self.kernels[kind].op, UserOp):
                 return self._user_instance(name)
             return f"// {kind} {name}({', '.join(args)}): launched at {name}.execute()"
         return None
+
+    def _acc(self, inst: str, name: str) -> AccInfo:
+        """Accessor `name` as it was when instance `inst` was declared."""
+        return self.inst_accs[inst][name]
 
     def _binding(self, inst: str) -> Tuple[KernelClass, Dict[str, str], str]:
         kname, args = self.instances[inst]
@@ -370,8 +505,8 @@ class Emitter:
         for p, a in zip(k.params, args):
             if p in k.param_field:
                 bind[k.param_field[p]] = a
-        iter_name = args[0]
-        if iter_name not in self.accessors:
+        iter_name = args[0] if args else ""
+        if iter_name not in self.inst_accs.get(inst, {}):
             raise EmitError(f"{inst}: the first argument is not an IterationSpace")
         return k, bind, iter_name
 
@@ -383,7 +518,7 @@ class Emitter:
         if isinstance(op, PointOp):
             ins = [bind[a] for a in op.inputs]
             if op.op in ("COPY", "AFFINE"):
-                t0 = self.images[self.accessors[ins[0]][0]]
+                t0 = self._img_type(self._acc(inst, ins[0]).img)
                 ab = ("" if op.op == "COPY" else
                       f", static_cast<const HipaccAccessor<{t0}>*>(nullptr), {op.alpha}, {op.beta}")
                 return f"hipaccLaunchPoint(HIPACC_POINT_{op.op}, {ins[0]}, {it}{ab});"
@@ -396,8 +531,8 @@ class Emitter:
 
     def _local_setup(self, inst: str, op: LocalOp, bind: Dict[str, str]):
         """Declarations of a local launch's options and mask."""
-        acc = bind[op.accessor]
-        img, mode, value = self.accessors[acc]
+        a = self._acc(inst, bind[op.accessor])
+        mode, value = a.mode, a.value
         mtype = "float" if op.acc_type in ("float", "double") else "int"
         o = f"{inst}_opt"
         lines = [f"HipaccLocalOptions {o};",
@@ -433,7 +568,9 @@ class Emitter:
             elif kind in ("IterationSpace", "Image", "BoundaryCondition"):
                 raise EmitError(f"kernel class {k.name}: member {name} of type {kind}")
             else:
-                members.append(f"{self.field_types[(k.name, name)]} {name};")
+                # scalar members are set from the host after construction
+                ty = re.sub(r"^const\s+|\s*&$", "", self.field_types[(k.name, name)].strip())
+                members.append(f"{ty} {name};")
         lines = [f"struct {k.name} : hipacc_dev::Kernel<{k.out_type}> {{"]
         lines += ["    " + m for m in members]
         lines.append("    __device__ void set_pos(int x_, int y_) { " + " ".join(pos) + " }")
@@ -442,16 +579,17 @@ class Emitter:
         return "\n".join(lines)
 
     def _user_instance(self, inst: str) -> str:
-        k, bind, _ = self._binding(inst)
+        k, bind, it = self._binding(inst)
         out = [f"{k.name} {inst};"]
-        for name, (kind, _) in k.fields.items():
+        for name, (kind, elem) in k.fields.items():
             if name not in bind:
                 raise EmitError(f"{inst}: member {name} is not set by the constructor")
             arg = bind[name]
             if kind == "Accessor":
-                img, mode, value = self.accessors[arg]
-                out.append(f"{inst}.{name} = hipaccDevAccessor({arg}, HipaccBoundary::{mode}, "
-                           f"{value});")
+                a = self._acc(inst, arg)
+                ip = f", HIPACC_INTERP_{a.interp}, &{it}" if a.interp else ""
+                out.append(f"{inst}.{name} = hipaccDevAccessor({arg}, HipaccBoundary::{a.mode}, "
+                           f"{a.value}{ip});")
             elif kind == "Mask":
                 out.append(f"{inst}.{name} = hipaccDevMask({arg});")
             elif kind == "Domain":
@@ -484,21 +622,24 @@ class Emitter:
         if a_prev is None or a_next is None:
             return False
         _, _, it = self._binding(prev)
-        x = self.accessors[it][0]
-        if self.accessors[a_next][0] != x or self.iter_crop.get(it) or self.iter_crop.get(a_next):
+        k, bind, it_next = self._binding(nxt)
+        acc_it, acc_next = self._acc(prev, it), self._acc(nxt, a_next)
+        x = acc_it.img
+        if acc_next.img != x or acc_it.crop or acc_next.crop:
             return False
         if self.images.get(x) not in self._FUSED_TYPES:
             return False
-        k, bind, it_next = self._binding(nxt)
         for other in (k.op.inputs[1:] if isinstance(k.op, PointOp) else []):
-            if self.accessors[bind[other]][0] == x:
+            if self._acc(nxt, bind[other]).img == x:
                 return False
-        sizes = {tuple(self.image_args[self.accessors[n][0]]) for n in (a_prev, it, a_next, it_next)}
-        if len(sizes) != 1:
+        imgs = [self._acc(prev, a_prev).img, x, self._acc(nxt, it_next).img]
+        if any(i not in self.image_args for i in imgs):
+            return False
+        if len({tuple(self.image_args[i]) for i in imgs}) != 1:
             return False
         # the image, its iteration space and the accessor (and boundary
         # condition) reading it are the only names of it in main()
-        decls = 1 + sum(1 for src in self.acc_src.values() if src == x)
+        decls = 1 + sum(1 for a in self.accessors.values() if a.src == x)
         decls += sum(1 for n, (img, _, _) in self.bcs.items() if img == x)
         if len(re.findall(r"\b%s\b" % re.escape(x), self.main_text)) != decls:
             return False
@@ -508,13 +649,16 @@ class Emitter:
     def _chains(self) -> List[List[Tuple[str, int, int]]]:
         chains: List[List[Tuple[str, int, int]]] = []
         for launch in self.launches:
-            if chains and self._extends(chains[-1], launch[0]):
+            if chains and self._extends(chains[-1], launch):
                 chains[-1].append(launch)
             else:
                 chains.append([launch])
         return chains
 
-    def _extends(self, chain, inst: str) -> bool:
+    def _extends(self, chain, launch: Tuple[str, int, int]) -> bool:
+        inst = launch[0]
+        if self.launch_scope[chain[-1][1]] != self.launch_scope[launch[1]]:
+            return False
         if len(chain) >= 8 or not self._fusible(chain[-1][0], inst):
             return False
         locals_ = sum(isinstance(self._binding(c[0])[0].op, LocalOp) for c in chain)
@@ -553,8 +697,8 @@ class Emitter:
             k, _, it = self._binding(name)
             if not k.binning_text:
                 raise EmitError(f"{name}.binned_data(): {k.name} has no binning()")
-            rep = (f"hipaccBinUser<{k.name}_bins, {k.bin_type}>({k.name}_bins{{}}, {it}, "
-                   f"{m.group(2).strip()})")
+            rep = self._run_first(name, f"hipaccBinUser<{k.name}_bins, {k.bin_type}>("
+                                        f"{k.name}_bins{{}}, {it}, {m.group(2).strip()})")
             s0 = b + len(text[:m.start()].encode())
             s1 = b + len(text[:m.end()].encode())
             if not any(x0 <= s0 < x1 for x0, x1, _ in self.edits):
@@ -571,6 +715,7 @@ class Emitter:
                     rep = f"hipaccApplyReduction<{k.out_type}>({it}, HipaccReduce::{k.global_reduce})"
                 else:
                     raise EmitError(f"{name}.reduced_data(): {k.name} has no reduce()")
+                rep = self._run_first(name, rep)
             else:
                 continue
             s0 = b + len(text[:m.start()].encode())
@@ -578,13 +723,23 @@ class Emitter:
             if not any(x0 <= s0 < x1 for x0, x1, _ in self.edits):
                 self.edits.append((s0, s1, rep))
 
+    def _run_first(self, inst: str, expr: str) -> str:
+        """reduced_data() / binned_data() of a kernel never executed: the DSL
+        executes it first (dsl/kernel.hpp:121-125,162-166)."""
+        if any(l[0] == inst for l in self.launches):
+            return expr
+        return f"[&] {{ {self._launch(inst)} return {expr}; }}()"
+
     def _apply(self) -> str:
         out = self.src
         for b, e, rep in sorted(self.edits, key=lambda t: t[0], reverse=True):
             out = out[:b] + rep.encode() + out[e:]
         text = out.decode()
+        # 4-wide pixel types need HIP's vector types (a hipcc build).
+        vec = re.search(r"\b(u?char|u?short|u?int|u?long|float|double)4\b", text)
         prologue = (("#include \"hipacc_hip_device.hpp\"\n" if self.user else
                      "#include \"hipacc_hip.hpp\"\n") +
+                    ("#include \"hipacc_hip_vec.hpp\"\n" if vec and not self.user else "") +
                     "// DSL names kept by the emitted host code\n"
                     "typedef unsigned char uchar;\n"
                     "typedef unsigned short ushort;\n"

@@ -189,13 +189,19 @@ class ParseError(RuntimeError):
     pass
 
 
+# CXTranslationUnit_DetailedPreprocessingRecord: macro definitions become
+# cursors ("macro definition").
+DETAILED_PREPROCESSING = 0x01
+
+
 class TranslationUnit:
-    def __init__(self, path: str, args: Sequence[str]):
+    def __init__(self, path: str, args: Sequence[str], options: int = 0):
         L = lib()
         self.index = L.clang_createIndex(0, 0)
         argv = [a.encode() for a in args]
         arr = (c_char_p * len(argv))(*argv)
-        self.tu = L.clang_parseTranslationUnit(self.index, path.encode(), arr, len(argv), None, 0, 0)
+        self.tu = L.clang_parseTranslationUnit(self.index, path.encode(), arr, len(argv), None, 0,
+                                               options)
         if not self.tu:
             raise ParseError(f"libclang could not parse {path}")
         errors = []

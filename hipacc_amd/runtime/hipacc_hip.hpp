@@ -229,6 +229,21 @@ struct HipaccMask {
   void domain_from_coef() {
     for (size_t i = 0; i < coef.size(); ++i) domain[i] = coef[i] != M(0);
   }
+  // Domain(array): enabled where the row-major size_y x size_x array is
+  // nonzero (dsl/mask.hpp:155-164).
+  template <typename E> void domain_write(const E* host) {
+    for (size_t i = 0; i < domain.size(); ++i) domain[i] = host[i] != E(0);
+  }
+  // Element (xf, yf), offsets from the centre, of the domain and of the
+  // coefficients: the DSL's dom(xf, yf) = v and mask(xf, yf) = v, at row
+  // size_y / 2 + yf and column size_x / 2 + xf as the reference compiler
+  // places a domain assignment (lib/Rewrite/Rewrite.cpp:1650-1678).
+  uint8_t& domain_at(int xf, int yf) {
+    return domain.at((size_t)(yf + size_y / 2) * size_x + xf + size_x / 2);
+  }
+  M& coef_at(int xf, int yf) {
+    return coef.at((size_t)(yf + size_y / 2) * size_x + xf + size_x / 2);
+  }
 };
 
 template <typename M>

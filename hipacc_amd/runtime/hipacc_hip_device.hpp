@@ -23,10 +23,12 @@
 
 #include <algorithm>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
-#include "../csrc/hh_common.hpp"
+#include "../csrc/hh_pyramid.hpp"
 #include "hipacc_hip.hpp"
+#include "hipacc_hip_vec.hpp"
 
 namespace hipacc_dev {
 
@@ -34,15 +36,28 @@ enum class Reduce : int { SUM = 0, MIN, MAX, PROD, MEDIAN };
 
 template <typename T> __device__ __forceinline__ T dsl_min(T a, T b) { return a < b ? a : b; }
 template <typename T> __device__ __forceinline__ T dsl_max(T a, T b) { return a > b ? a : b; }
+// 4-wide pixels: componentwise (dsl/math_functions.hpp:345-392).
+template <typename E>
+__device__ __forceinline__ HIP_vector_type<E, 4> dsl_min(HIP_vector_type<E, 4> a,
+                                                         HIP_vector_type<E, 4> b) {
+  return hipacc::math::min(a, b);
+}
+template <typename E>
+__device__ __forceinline__ HIP_vector_type<E, 4> dsl_max(HIP_vector_type<E, 4> a,
+                                                         HIP_vector_type<E, 4> b) {
+  return hipacc::math::max(a, b);
+}
 
-// An accessor: the image window it reads, its boundary handling, and the
-// iteration-space pixel (px, py) the kernel is at.
+// An accessor: the image window it reads, its boundary handling, its
+// interpolation, and the iteration-space pixel (px, py) the kernel is at.
 template <typename T> struct Accessor {
   const T* img = nullptr;
   int stride = 0, img_w = 0, img_h = 0;
   int lo_x = 0, lo_y = 0, w = 0, h = 0;
   int mode = HIPACC_BOUNDARY_UNDEFINED;
   T cval{};
+  int interp = 0;                 // 0 or HIPACC_INTERP_NN ... L3
+  float isx = 1.0f, isy = 1.0f;   // window extent / iteration-space extent
   int px = 0, py = 0;
 
   // Pixel (x, y) of the accessor's window (window coordinates), through the
@@ -57,10 +72,30 @@ template <typename T> struct Accessor {
     }
     return img[(long long)ay * stride + ax];
   }
-  __device__ __forceinline__ T operator()() const { return at(px, py); }
-  __device__ __forceinline__ T operator()(int xf, int yf) const { return at(px + xf, py + yf); }
+  // Iteration-space pixel (x, y) through the interpolation, if any
+  // (dsl/image.hpp:390-532): mapped into the window (cell-centred, scale
+  // window / iteration-space extent); NN returns the pixel there, the other
+  // modes their float value converted to T with a C cast.
+  __device__ T read(int x, int y) const {
+    if (interp == 0) return at(x, y);
+    const float mx = hh::interp_map(isx, x, lo_x), my = hh::interp_map(isy, y, lo_y);
+    if (interp == HIPACC_INTERP_NN) return at((int)mx - lo_x, (int)my - lo_y);
+    if constexpr (std::is_arithmetic<T>::value) {
+      auto rd = [&](int ax, int ay) { return (float)at(ax - lo_x, ay - lo_y); };
+      switch (interp) {
+        case HIPACC_INTERP_LF: return (T)hh::interp_sample<HIPACC_INTERP_LF>(rd, mx, my);
+        case HIPACC_INTERP_B5: return (T)hh::interp_sample<HIPACC_INTERP_B5>(rd, mx, my);
+        case HIPACC_INTERP_CF: return (T)hh::interp_sample<HIPACC_INTERP_CF>(rd, mx, my);
+        default: return (T)hh::interp_sample<HIPACC_INTERP_L3>(rd, mx, my);
+      }
+    } else {
+      return at(x, y);    // 4-wide pixels: the host allows NN only
+    }
+  }
+  __device__ __forceinline__ T operator()() const { return read(px, py); }
+  __device__ __forceinline__ T operator()(int xf, int yf) const { return read(px + xf, py + yf); }
   template <typename M> __device__ __forceinline__ T operator()(const M& m) const {
-    return at(px + m.x(), py + m.y());
+    return read(px + m.x(), py + m.y());
   }
   __device__ __forceinline__ T pixel_at(int x, int y) const {
     return img[(long long)(lo_y + y) * stride + lo_x + x];
@@ -223,6 +258,54 @@ __global__ __launch_bounds__(256) void user_binning_kernel(F f, const T* img, in
     if (lb[b] != TB{}) combine_at(k, bins + b, lb[b]);
 }
 
+// Bins of values wider than 32 bits (no atomic update of such a value):
+// one wave per block; the wave's lanes that have a bin update the block's
+// LDS bins one lane at a time, so no two updates of a bin overlap; each
+// block stores its bins as one row of `rows`, and user_binning_fold then
+// folds the rows bin by bin in block order.
+template <typename F, typename T, typename TB>
+__global__ __launch_bounds__(64) void user_binning_wide_kernel(F f, const T* img, int stride,
+                                                               int ox, int oy, int w, int h,
+                                                               TB* rows, unsigned nbins) {
+  extern __shared__ __align__(16) unsigned char user_bins_lds[];
+  TB* lb = reinterpret_cast<TB*>(user_bins_lds);
+  const int lane = threadIdx.x;
+  for (unsigned b = lane; b < nbins; b += 64) lb[b] = TB{};
+  __syncthreads();
+  F k = f;
+  k.nbins_ = nbins;
+  const long long n = (long long)w * h;
+  for (long long base = (long long)blockIdx.x * 64; base < n; base += (long long)gridDim.x * 64) {
+    const long long i = base + lane;
+    bool have = false;
+    if (i < n) {
+      const int y = (int)(i / w), x = (int)(i - (long long)y * w);
+      k.bidx_ = 0xffffffffu;
+      k.binning((unsigned)x, (unsigned)y, img[(long long)(oy + y) * stride + ox + x]);
+      have = k.bidx_ < nbins;
+    }
+    unsigned long long pending = __ballot(have);
+    while (pending) {
+      const int l = __ffsll((long long)pending) - 1;
+      pending &= pending - 1;
+      if (lane == l) lb[k.bidx_] = k.reduce(lb[k.bidx_], k.bval_);
+    }
+  }
+  __syncthreads();
+  for (unsigned b = lane; b < nbins; b += 64) rows[(long long)blockIdx.x * nbins + b] = lb[b];
+}
+
+template <typename F, typename TB>
+__global__ __launch_bounds__(256) void user_binning_fold(F f, const TB* rows, int nrows,
+                                                         unsigned nbins, TB* bins) {
+  const unsigned b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= nbins) return;
+  const F k = f;
+  TB acc = rows[b];
+  for (int r = 1; r < nrows; ++r) acc = k.reduce(acc, rows[(long long)r * nbins + b]);
+  bins[b] = acc;
+}
+
 // A user reduction reduce(l, r) over an image window: per-block trees in
 // LDS, then one block folds the per-block results, both in a fixed order.
 template <typename F, typename T>
@@ -270,9 +353,13 @@ __global__ __launch_bounds__(64) void user_reduce_final(F f, T* partial, int npa
 // ---- host side ---------------------------------------------------------------
 // The device view of a runtime accessor with the boundary handling of its
 // BoundaryCondition.
+// With an interpolation (HIPACC_INTERP_NN ... L3; 0 for none), `is` is the
+// iteration space the accessor is read over: its extent against the
+// accessor's sets the mapping.
 template <typename T>
 hipacc_dev::Accessor<T> hipaccDevAccessor(const HipaccAccessor<T>& acc, HipaccBoundary mode,
-                                          double const_val = 0.0) {
+                                          double const_val = 0.0, int interp = 0,
+                                          const HipaccAccessorBase* is = nullptr) {
   const hipacc_view v = hipaccView(acc);
   hipacc_dev::Accessor<T> d;
   d.img = static_cast<const T*>(v.data);
@@ -285,6 +372,18 @@ hipacc_dev::Accessor<T> hipaccDevAccessor(const HipaccAccessor<T>& acc, HipaccBo
   d.h = v.height;
   d.mode = (int)mode;
   d.cval = (T)const_val;
+  if (interp) {
+    if ((!std::is_arithmetic<T>::value && interp != HIPACC_INTERP_NN) || !is ||
+        is->width <= 0 || is->height <= 0) {
+      hipaccRuntimeLogTrivial(hipaccRuntimeLogLevel::ERROR,
+                              "ERROR: hipaccDevAccessor: interpolation needs the iteration "
+                              "space, and a scalar pixel type unless it is NN");
+    } else {
+      d.interp = interp;
+      d.isx = (float)v.width / (float)is->width;
+      d.isy = (float)v.height / (float)is->height;
+    }
+  }
   return d;
 }
 
@@ -318,18 +417,51 @@ template <typename M> hipacc_dev::Domain hipaccDevDomain(const HipaccMask<M>& m)
   return d;
 }
 
+// Times the launches made here (not in the library) when the calling
+// thread's timing is on, as the library's entry points time theirs: events
+// around the launches on their stream, a synchronisation, and the elapsed
+// time handed to hipacc_last_kernel_timing().
+class HipaccLaunchTimer {
+  hipStream_t s_;
+  hipEvent_t e0_ = nullptr, e1_ = nullptr;
+
+ public:
+  explicit HipaccLaunchTimer(hipStream_t s) : s_(s) {
+    if (hipacc_hip_get_timing() && hipEventCreate(&e0_) == hipSuccess &&
+        hipEventCreate(&e1_) == hipSuccess)
+      hipEventRecord(e0_, s_);
+  }
+  void finish(int status) {
+    if (e0_ && e1_ && status == HIPACC_OK) {
+      float ms = 0.0f;
+      if (hipEventRecord(e1_, s_) == hipSuccess && hipEventSynchronize(e1_) == hipSuccess &&
+          hipEventElapsedTime(&ms, e0_, e1_) == hipSuccess)
+        (void)hipacc_hip_set_last_kernel_timing(ms);
+    }
+  }
+  ~HipaccLaunchTimer() {
+    if (e0_) (void)hipEventDestroy(e0_);
+    if (e1_) (void)hipEventDestroy(e1_);
+  }
+  HipaccLaunchTimer(const HipaccLaunchTimer&) = delete;
+  HipaccLaunchTimer& operator=(const HipaccLaunchTimer&) = delete;
+};
+
 // Launches a user kernel over the iteration space (a DSL image: one plane).
 template <typename F, typename TO>
 int hipaccLaunchUser(const F& f, const HipaccAccessor<TO>& is,
                      const HipaccExecutionParameterHip& ep = nullptr) {
   const hipacc_view v = hipaccView(is);
   dim3 grid((v.width + 63) / 64, (v.height + 3) / 4, 1);
+  hipStream_t s = static_cast<hipStream_t>(hipaccStreamOf(ep));
   if (ep) ep->pre_kernel();
-  hipLaunchKernelGGL((hipacc_dev::user_kernel<F, TO>), grid, dim3(64, 4), 0,
-                     static_cast<hipStream_t>(hipaccStreamOf(ep)), f, static_cast<TO*>(v.data),
-                     v.stride, v.offset_x, v.offset_y, v.width, v.height);
+  HipaccLaunchTimer timer(s);
+  hipLaunchKernelGGL((hipacc_dev::user_kernel<F, TO>), grid, dim3(64, 4), 0, s, f,
+                     static_cast<TO*>(v.data), v.stride, v.offset_x, v.offset_y, v.width,
+                     v.height);
   const hipError_t e = hipGetLastError();
   const int st = hipaccCheck(e == hipSuccess ? HIPACC_OK : HIPACC_ERR_HIP, "hipaccLaunchUser");
+  timer.finish(st);
   if (ep) ep->post_kernel();
   return st;
 }
@@ -341,21 +473,45 @@ TB* hipaccBinUser(const F& f, const HipaccAccessor<T>& is, unsigned num_bins,
                   const HipaccExecutionParameterHip& ep = nullptr) {
   TB* result = new TB[num_bins]();
   const hipacc_view v = hipaccView(is);
+  const size_t lds = (size_t)num_bins * sizeof(TB);
+  if (num_bins == 0 || lds > 65536) {
+    hipaccRuntimeLogTrivial(hipaccRuntimeLogLevel::ERROR,
+                            "ERROR: hipaccBinUser: " + std::to_string(num_bins) +
+                                " bins do not fit the 64 KiB of a block's bins");
+    return result;
+  }
+  constexpr bool narrow = sizeof(TB) == 4;
+  const long long n = (long long)v.width * v.height;
+  const int blocks = (int)std::min<long long>(std::max<long long>((n + 4095) / 4096, 1),
+                                              narrow ? 1024 : 512);
+  // the bins, and for wide values the per-block rows after them
+  const size_t words = (lds + 3) / 4 + (narrow ? 0 : (size_t)blocks * lds / 4);
   void* bins = nullptr;
   int32_t stride = 0;
-  if (hipaccCheck(hipacc_hip_alloc2d(&bins, &stride, (int32_t)num_bins, 1, 4, 0), "hipaccBinUser"))
+  if (hipaccCheck(hipacc_hip_alloc2d(&bins, &stride, (int32_t)words, 1, 4, 0), "hipaccBinUser"))
     return result;
   hipStream_t s = static_cast<hipStream_t>(hipaccStreamOf(ep));
-  hipaccCheck(hipacc_hip_fill(bins, 0u, num_bins, 4, s), "hipaccBinUser");
-  const long long n = (long long)v.width * v.height;
-  const int blocks = (int)std::min<long long>(std::max<long long>((n + 4095) / 4096, 1), 1024);
+  if (narrow) hipaccCheck(hipacc_hip_fill(bins, 0u, num_bins, 4, s), "hipaccBinUser");
   if (ep) ep->pre_kernel();
-  hipLaunchKernelGGL((hipacc_dev::user_binning_kernel<F, T, TB>), dim3(blocks), dim3(256),
-                     (size_t)num_bins * sizeof(TB), s, f, static_cast<const T*>(v.data), v.stride,
-                     v.offset_x, v.offset_y, v.width, v.height, static_cast<TB*>(bins), num_bins);
+  HipaccLaunchTimer timer(s);
+  TB* out = static_cast<TB*>(bins);
+  if constexpr (narrow) {
+    hipLaunchKernelGGL((hipacc_dev::user_binning_kernel<F, T, TB>), dim3(blocks), dim3(256), lds,
+                       s, f, static_cast<const T*>(v.data), v.stride, v.offset_x, v.offset_y,
+                       v.width, v.height, out, num_bins);
+  } else {
+    TB* rows = reinterpret_cast<TB*>(static_cast<char*>(bins) + (lds + 3) / 4 * 4);
+    hipLaunchKernelGGL((hipacc_dev::user_binning_wide_kernel<F, T, TB>), dim3(blocks), dim3(64),
+                       lds, s, f, static_cast<const T*>(v.data), v.stride, v.offset_x,
+                       v.offset_y, v.width, v.height, rows, num_bins);
+    hipLaunchKernelGGL((hipacc_dev::user_binning_fold<F, TB>), dim3((num_bins + 255) / 256),
+                       dim3(256), 0, s, f, rows, blocks, num_bins, out);
+  }
   const hipError_t e = hipGetLastError();
-  if (!hipaccCheck(e == hipSuccess ? HIPACC_OK : HIPACC_ERR_HIP, "hipaccBinUser"))
-    hipaccCheck(hipacc_hip_download2d(result, bins, (int32_t)num_bins, (int32_t)num_bins, 1, 4, s),
+  const int st = hipaccCheck(e == hipSuccess ? HIPACC_OK : HIPACC_ERR_HIP, "hipaccBinUser");
+  timer.finish(st);
+  if (!st)
+    hipaccCheck(hipacc_hip_download2d(result, bins, (int32_t)lds, (int32_t)lds, 1, 1, s),
                 "hipaccBinUser");
   if (ep) ep->post_kernel();
   (void)hipacc_hip_free(bins);
@@ -379,13 +535,16 @@ T hipaccReduceUser(const F& f, const HipaccAccessor<T>& is,
   const long long n = (long long)v.width * v.height;
   const int blocks = (int)std::min<long long>(std::max<long long>((n + 255) / 256, 1), kBlocks);
   if (ep) ep->pre_kernel();
+  HipaccLaunchTimer timer(s);
   hipLaunchKernelGGL((hipacc_dev::user_reduce_kernel<F, T>), dim3(blocks), dim3(256), 0, s, f,
                      static_cast<const T*>(v.data), v.stride, v.offset_x, v.offset_y, v.width,
                      v.height, static_cast<T*>(part));
   hipLaunchKernelGGL((hipacc_dev::user_reduce_final<F, T>), dim3(1), dim3(64), 0, s, f,
                      static_cast<T*>(part), blocks);
   const hipError_t e = hipGetLastError();
-  if (!hipaccCheck(e == hipSuccess ? HIPACC_OK : HIPACC_ERR_HIP, "hipaccReduceUser"))
+  const int st = hipaccCheck(e == hipSuccess ? HIPACC_OK : HIPACC_ERR_HIP, "hipaccReduceUser");
+  timer.finish(st);
+  if (!st)
     hipaccCheck(hipacc_hip_download2d(&result, part, 1, 1, 1, (int32_t)sizeof(T), s),
                 "hipaccReduceUser");
   if (ep) ep->post_kernel();

@@ -560,7 +560,8 @@ int hipacc_hip_init(int device);
 const char* hipacc_hip_strerror(int status);
 const char* hipacc_hip_last_error(void);
 /* hipaccCreateMemory (runtime/hipacc_cu.tpp:33-83): pitched allocation, row
- * pitch rounded up to `alignment` bytes (0 = 256).                          */
+ * pitch rounded up to `alignment` bytes (0 = 256); pixels of 1, 2, 4, 8 or
+ * 16 bytes (the 4-wide pixel types up to float4 / int4).                    */
 int hipacc_hip_alloc2d(void** ptr, int32_t* stride_elems, int32_t width,
                        int32_t height, int32_t elem_bytes, int32_t alignment);
 int hipacc_hip_free(void* ptr);
@@ -594,6 +595,12 @@ int hipacc_hip_fill_random(const hipacc_view* img, uint64_t seed, void* stream);
  * call that fails leaves the last time unchanged.                          */
 int   hipacc_hip_set_timing(int enabled);
 float hipacc_hip_last_kernel_timing(void);
+/* The calling thread's timing switch, and a time recorded by a launch made
+ * outside the library (kernels the C++ runtime compiles from a program's
+ * own kernel() code), so that hipacc_last_kernel_timing() reports it:
+ * HipaccKernelTimingBase::set_timing (runtime/hipacc_base.hpp:54-66).      */
+int   hipacc_hip_get_timing(void);
+int   hipacc_hip_set_last_kernel_timing(float ms);
 
 /* Library version string, e.g. "hipacc-gfx950 0.1". */
 const char* hipacc_hip_version(void);

@@ -130,3 +130,37 @@ def test_fusible_chain_is_one_launch():
     separate = (DSL / "emitted" / "point_f32.cpp").read_text()
     assert "hipaccLaunchFused(" not in separate
     assert separate.count("hipaccLaunchPoint(") == 3 and separate.count("hipaccLaunchLocal(") == 1
+
+
+SAMPLES = Path("/root/reference/samples-public")
+
+
+def _samples():
+    return sorted(p.parent.parent.name for p in SAMPLES.glob("[0-6]_*/*/src/main.cpp"))
+
+
+@pytest.mark.skipif(not _clang_ok() or not SAMPLES.exists(),
+                    reason="libclang or the reference's samples are not available")
+@pytest.mark.parametrize("name", _samples())
+def test_emitter_takes_every_reference_sample(name):
+    """Every sample program of the reference is emitted unchanged (the GPU
+    test tests/test_gpu_samples.py runs them); a few structural facts of the
+    rewriting are checked here."""
+    from hipacc_amd.emit import emit
+    src = next(SAMPLES.glob(f"*/{name}/src/main.cpp"))
+    text = emit(str(src), [DSL_HEADERS, str(SAMPLES / "common")])
+    assert "hipaccInitHIP(0);" in text
+    main = text.split("HIPACC_CODEGEN int main", 1)[1]
+    assert not re.search(r"\b(Image|Accessor|IterationSpace|BoundaryCondition|Pyramid)<", main)
+    if name == "Gaussian_Laplacian_Pyramid":
+        assert "HipaccPyramidTraversor hipaccTraversor;" in text
+        assert text.count("hipaccTraversor.hipaccTraverse") == 2
+        assert "HIPACC_INTERP_NN, &iter2" in text and "HIPACC_INTERP_LF" in text
+        assert "hipaccLaunchLocal(acc1, iter1" in text     # the Gaussian stays a library call
+    if name == "Game_of_Life":
+        assert "dom.domain_at(0,0) = 0;" in text and "hipaccCopyMemory(out, in);" in text
+    if name == "Motion_Interpolation":
+        assert "hipaccWriteMemory<float4>(merged_vec, vecs);" in text
+        assert "[&] { hipaccLaunchPoint(HIPACC_POINT_COPY, acc_vec, iter_vec);" in text
+    if name.startswith("Kernel_Fusion_"):        # the whole chain in one launch
+        assert text.count("hipaccLaunchFused(") == 1 and "hipaccLaunchUser" not in text

@@ -56,5 +56,14 @@ def test_timing_on_off_failure_and_threads(gpu):
         th.join()
         assert seen[0] == 0.0 and seen[1] == 0.0 and seen[2] > 0.0, seen
         assert L.hipacc_hip_last_kernel_timing() == t1   # this thread's value
+        # the switch as the runtime reads it, and a time recorded for a
+        # launch made outside the library (a negative one is refused)
+        assert L.hipacc_hip_get_timing() == 0
+        L.hipacc_hip_set_timing(1)
+        assert L.hipacc_hip_get_timing() == 1
+        assert L.hipacc_hip_set_last_kernel_timing(ctypes.c_float(2.5)) == 0
+        assert L.hipacc_hip_last_kernel_timing() == 2.5
+        assert L.hipacc_hip_set_last_kernel_timing(ctypes.c_float(-1.0)) == 1
+        assert L.hipacc_hip_last_kernel_timing() == 2.5
     finally:
         L.hipacc_hip_set_timing(0)
