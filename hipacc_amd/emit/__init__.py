@@ -296,11 +296,14 @@ class Emitter:
             text = f"// Kernel class {k.name}: {_describe(k)}"
             if isinstance(k.op, UserOp):
                 text += "\n" + self._user_struct(k)
+            scalars = "".join(f"    {ty} {n};\n" for n, ty in self._scalar_members(k))
             if k.binning_text:
                 text += (f"\nstruct {k.name}_bins : hipacc_dev::Binning<{k.bin_type}> {{\n"
-                         f"    __device__ {k.binning_text}\n    __device__ {k.reduce_text}\n}};")
+                         f"{scalars}    __device__ {k.binning_text}\n"
+                         f"    __device__ {k.reduce_text}\n}};")
             elif k.global_reduce == "USER":
-                text += (f"\nstruct {k.name}_reduce {{\n    __device__ {k.reduce_text}\n}};")
+                text += (f"\nstruct {k.name}_reduce {{\n{scalars}"
+                         f"    __device__ {k.reduce_text}\n}};")
             self.edits.append((b, e, text))
         self.user = any(isinstance(k.op, UserOp) or k.binning_text or k.global_reduce == "USER"
                         for k in self.kernels.values())
@@ -569,8 +572,7 @@ class Emitter:
                 raise EmitError(f"kernel class {k.name}: member {name} of type {kind}")
             else:
                 # scalar members are set from the host after construction
-                ty = re.sub(r"^const\s+|\s*&$", "", self.field_types[(k.name, name)].strip())
-                members.append(f"{ty} {name};")
+                members.append(f"{dict(self._scalar_members(k))[name]} {name};")
         lines = [f"struct {k.name} : hipacc_dev::Kernel<{k.out_type}> {{"]
         lines += ["    " + m for m in members]
         lines.append("    __device__ void set_pos(int x_, int y_) { " + " ".join(pos) + " }")
@@ -698,7 +700,8 @@ class Emitter:
             if not k.binning_text:
                 raise EmitError(f"{name}.binned_data(): {k.name} has no binning()")
             rep = self._run_first(name, f"hipaccBinUser<{k.name}_bins, {k.bin_type}>("
-                                        f"{k.name}_bins{{}}, {it}, {m.group(2).strip()})")
+                                        f"{self._functor(name, '_bins')}, {it}, "
+                                        f"{m.group(2).strip()})")
             s0 = b + len(text[:m.start()].encode())
             s1 = b + len(text[:m.end()].encode())
             if not any(x0 <= s0 < x1 for x0, x1, _ in self.edits):
@@ -710,7 +713,7 @@ class Emitter:
             elif what == "reduced_data" and name in self.instances:
                 k, _, it = self._binding(name)
                 if k.global_reduce == "USER":
-                    rep = f"hipaccReduceUser({k.name}_reduce{{}}, {it})"
+                    rep = f"hipaccReduceUser({self._functor(name, '_reduce')}, {it})"
                 elif k.global_reduce:
                     rep = f"hipaccApplyReduction<{k.out_type}>({it}, HipaccReduce::{k.global_reduce})"
                 else:
@@ -723,6 +726,26 @@ class Emitter:
             if not any(x0 <= s0 < x1 for x0, x1, _ in self.edits):
                 self.edits.append((s0, s1, rep))
 
+    def _scalar_members(self, k: KernelClass) -> List[Tuple[str, str]]:
+        """(name, type) of a kernel class's members other than accessors,
+        masks and domains, without const or reference."""
+        out = []
+        for name, (kind, _) in k.fields.items():
+            if kind not in ("Accessor", "Mask", "Domain", "IterationSpace", "Image",
+                            "BoundaryCondition"):
+                ty = re.sub(r"^const\s+|\s*&$", "", self.field_types[(k.name, name)].strip())
+                out.append((name, ty))
+        return out
+
+    def _functor(self, inst: str, suffix: str) -> str:
+        """The binning / reduce struct of an instance, its scalar members set
+        from the instance's constructor arguments."""
+        k, bind, _ = self._binding(inst)
+        sets = [f"f.{n} = {bind[n]};" for n, _ in self._scalar_members(k) if n in bind]
+        if not sets:
+            return f"{k.name}{suffix}{{}}"
+        return f"[&] {{ {k.name}{suffix} f; {' '.join(sets)} return f; }}()"
+
     def _run_first(self, inst: str, expr: str) -> str:
         """reduced_data() / binned_data() of a kernel never executed: the DSL
         executes it first (dsl/kernel.hpp:121-125,162-166)."""
@@ -733,6 +756,11 @@ class Emitter:
     def _apply(self) -> str:
         out = self.src
         for b, e, rep in sorted(self.edits, key=lambda t: t[0], reverse=True):
+            # statements nested deeper than main()'s body: their continuation
+            # lines follow the statement's own indentation
+            line = out[out.rfind(b"\n", 0, b) + 1:b]
+            if len(line) > 4 and not line.strip():
+                rep = rep.replace("\n    ", "\n" + line.decode())
             out = out[:b] + rep.encode() + out[e:]
         text = out.decode()
         # 4-wide pixel types need HIP's vector types (a hipcc build).

@@ -429,7 +429,7 @@ class HipaccLaunchTimer {
   explicit HipaccLaunchTimer(hipStream_t s) : s_(s) {
     if (hipacc_hip_get_timing() && hipEventCreate(&e0_) == hipSuccess &&
         hipEventCreate(&e1_) == hipSuccess)
-      hipEventRecord(e0_, s_);
+      (void)hipEventRecord(e0_, s_);
   }
   void finish(int status) {
     if (e0_ && e1_ && status == HIPACC_OK) {
