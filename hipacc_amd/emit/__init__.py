@@ -279,13 +279,18 @@ class Emitter:
         self.main_text = self.src[mb:me].decode()
         # A kernel the library would run as an operator but whose instance
         # reads through an interpolating accessor is compiled as device code
-        # instead; main() is then walked again.
-        for _ in range(2):
+        # instead, and the accessor members some instance binds to an
+        # interpolating accessor are declared as such; main() is walked
+        # again until both are settled.
+        self.interp_fields: Dict[str, set] = {}
+        for _ in range(3):
             self._walk_main(body)
             forced = {self.instances[i][0] for i in self.interp_insts}
             forced = {n for n in forced if not isinstance(self.kernels[n].op, UserOp)}
-            if not forced:
+            fields = self._interp_fields()
+            if not forced and fields == self.interp_fields:
                 break
+            self.interp_fields = fields
             for n in forced:
                 k = self.kernels[n]
                 k.op = UserOp(k.body_text, "reads through an interpolating accessor")
@@ -495,6 +500,17 @@ class Emitter:
             return f"// {kind} {name}({', '.join(args)}): launched at {name}.execute()"
         return None
 
+    def _interp_fields(self) -> Dict[str, set]:
+        """Kernel class -> its accessor members bound to an interpolating
+        accessor by some instance."""
+        out: Dict[str, set] = {}
+        for inst in self.interp_insts:
+            k, bind, _ = self._binding(inst)
+            for f, (kind, _) in k.fields.items():
+                if kind == "Accessor" and f in bind and self._acc(inst, bind[f]).interp:
+                    out.setdefault(k.name, set()).add(f)
+        return out
+
     def _acc(self, inst: str, name: str) -> AccInfo:
         """Accessor `name` as it was when instance `inst` was declared."""
         return self.inst_accs[inst][name]
@@ -562,7 +578,8 @@ class Emitter:
         members, pos = [], ["it_x = x_;", "it_y = y_;"]
         for name, (kind, elem) in k.fields.items():
             if kind == "Accessor":
-                members.append(f"hipacc_dev::Accessor<{elem}> {name};")
+                ip = ", true" if name in self.interp_fields.get(k.name, ()) else ""
+                members.append(f"hipacc_dev::Accessor<{elem}{ip}> {name};")
                 pos.append(f"{name}.px = x_; {name}.py = y_;")
             elif kind == "Mask":
                 members.append(f"hipacc_dev::Mask<{elem}> {name};")
@@ -576,7 +593,7 @@ class Emitter:
         lines = [f"struct {k.name} : hipacc_dev::Kernel<{k.out_type}> {{"]
         lines += ["    " + m for m in members]
         lines.append("    __device__ void set_pos(int x_, int y_) { " + " ".join(pos) + " }")
-        lines.append("    __device__ void kernel() " + k.op.body)
+        lines.append("    __device__ __forceinline__ void kernel() " + k.op.body)
         lines.append("};")
         return "\n".join(lines)
 
@@ -590,8 +607,9 @@ class Emitter:
             if kind == "Accessor":
                 a = self._acc(inst, arg)
                 ip = f", HIPACC_INTERP_{a.interp}, &{it}" if a.interp else ""
-                out.append(f"{inst}.{name} = hipaccDevAccessor({arg}, HipaccBoundary::{a.mode}, "
-                           f"{a.value}{ip});")
+                tpl = "<true>" if name in self.interp_fields.get(k.name, ()) else ""
+                out.append(f"{inst}.{name} = hipaccDevAccessor{tpl}({arg}, "
+                           f"HipaccBoundary::{a.mode}, {a.value}{ip});")
             elif kind == "Mask":
                 out.append(f"{inst}.{name} = hipaccDevMask({arg});")
             elif kind == "Domain":

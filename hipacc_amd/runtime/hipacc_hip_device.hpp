@@ -50,7 +50,9 @@ __device__ __forceinline__ HIP_vector_type<E, 4> dsl_max(HIP_vector_type<E, 4> a
 
 // An accessor: the image window it reads, its boundary handling, its
 // interpolation, and the iteration-space pixel (px, py) the kernel is at.
-template <typename T> struct Accessor {
+// INTERP: the accessor may be bound to an interpolating window (the code of
+// the interpolation modes is compiled in only then).
+template <typename T, bool INTERP = false> struct Accessor {
   const T* img = nullptr;
   int stride = 0, img_w = 0, img_h = 0;
   int lo_x = 0, lo_y = 0, w = 0, h = 0;
@@ -76,8 +78,8 @@ template <typename T> struct Accessor {
   // (dsl/image.hpp:390-532): mapped into the window (cell-centred, scale
   // window / iteration-space extent); NN returns the pixel there, the other
   // modes their float value converted to T with a C cast.
-  __device__ T read(int x, int y) const {
-    if (interp == 0) return at(x, y);
+  __device__ __forceinline__ T read(int x, int y) const {
+    if (!INTERP || interp == 0) return at(x, y);
     const float mx = hh::interp_map(isx, x, lo_x), my = hh::interp_map(isy, y, lo_y);
     if (interp == HIPACC_INTERP_NN) return at((int)mx - lo_x, (int)my - lo_y);
     if constexpr (std::is_arithmetic<T>::value) {
@@ -356,12 +358,12 @@ __global__ __launch_bounds__(64) void user_reduce_final(F f, T* partial, int npa
 // With an interpolation (HIPACC_INTERP_NN ... L3; 0 for none), `is` is the
 // iteration space the accessor is read over: its extent against the
 // accessor's sets the mapping.
-template <typename T>
-hipacc_dev::Accessor<T> hipaccDevAccessor(const HipaccAccessor<T>& acc, HipaccBoundary mode,
+template <bool INTERP = false, typename T>
+hipacc_dev::Accessor<T, INTERP> hipaccDevAccessor(const HipaccAccessor<T>& acc, HipaccBoundary mode,
                                           double const_val = 0.0, int interp = 0,
                                           const HipaccAccessorBase* is = nullptr) {
   const hipacc_view v = hipaccView(acc);
-  hipacc_dev::Accessor<T> d;
+  hipacc_dev::Accessor<T, INTERP> d;
   d.img = static_cast<const T*>(v.data);
   d.stride = v.stride;
   d.img_w = v.img_width;
@@ -373,11 +375,12 @@ hipacc_dev::Accessor<T> hipaccDevAccessor(const HipaccAccessor<T>& acc, HipaccBo
   d.mode = (int)mode;
   d.cval = (T)const_val;
   if (interp) {
-    if ((!std::is_arithmetic<T>::value && interp != HIPACC_INTERP_NN) || !is ||
+    if (!INTERP || (!std::is_arithmetic<T>::value && interp != HIPACC_INTERP_NN) || !is ||
         is->width <= 0 || is->height <= 0) {
       hipaccRuntimeLogTrivial(hipaccRuntimeLogLevel::ERROR,
-                              "ERROR: hipaccDevAccessor: interpolation needs the iteration "
-                              "space, and a scalar pixel type unless it is NN");
+                              "ERROR: hipaccDevAccessor: interpolation needs an accessor "
+                              "made with INTERP, the iteration space, and a scalar pixel "
+                              "type unless it is NN");
     } else {
       d.interp = interp;
       d.isx = (float)v.width / (float)is->width;

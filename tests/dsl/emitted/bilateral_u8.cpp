@@ -27,7 +27,7 @@ struct BilateralFilter : hipacc_dev::Kernel<uchar> {
     hipacc_dev::Domain dom;
     int sigma_r;
     __device__ void set_pos(int x_, int y_) { it_x = x_; it_y = y_; in.px = x_; in.py = y_; }
-    __device__ void kernel() {
+    __device__ __forceinline__ void kernel() {
         float c_r = 0.5f / (sigma_r * sigma_r);
         float d = 0.0f;
         float p = 0.0f;

@@ -31,32 +31,32 @@ using namespace hipacc::math;
 
 // Kernel class Take: device code, the kernel() body compiled as written (hipaccLaunchUser)
 struct Take : hipacc_dev::Kernel<char> {
-    hipacc_dev::Accessor<char> in;
+    hipacc_dev::Accessor<char, true> in;
     __device__ void set_pos(int x_, int y_) { it_x = x_; it_y = y_; in.px = x_; in.py = y_; }
-    __device__ void kernel() { output() = in(); }
+    __device__ __forceinline__ void kernel() { output() = in(); }
 };
 
 // Kernel class Detail: device code, the kernel() body compiled as written (hipaccLaunchUser)
 struct Detail : hipacc_dev::Kernel<char> {
     hipacc_dev::Accessor<char> fine;
-    hipacc_dev::Accessor<char> coarse;
+    hipacc_dev::Accessor<char, true> coarse;
     __device__ void set_pos(int x_, int y_) { it_x = x_; it_y = y_; fine.px = x_; fine.py = y_; coarse.px = x_; coarse.py = y_; }
-    __device__ void kernel() { output() = fine() - coarse(); }
+    __device__ __forceinline__ void kernel() { output() = fine() - coarse(); }
 };
 
 // Kernel class Rebuild: device code, the kernel() body compiled as written (hipaccLaunchUser)
 struct Rebuild : hipacc_dev::Kernel<char> {
-    hipacc_dev::Accessor<char> coarse;
+    hipacc_dev::Accessor<char, true> coarse;
     hipacc_dev::Accessor<char> detail;
     __device__ void set_pos(int x_, int y_) { it_x = x_; it_y = y_; coarse.px = x_; coarse.py = y_; detail.px = x_; detail.py = y_; }
-    __device__ void kernel() { output() = coarse() + detail(); }
+    __device__ __forceinline__ void kernel() { output() = coarse() + detail(); }
 };
 
 // Kernel class Step: device code, the kernel() body compiled as written (hipaccLaunchUser)
 struct Step : hipacc_dev::Kernel<short> {
     hipacc_dev::Accessor<short> in;
     __device__ void set_pos(int x_, int y_) { it_x = x_; it_y = y_; in.px = x_; in.py = y_; }
-    __device__ void kernel() {
+    __device__ __forceinline__ void kernel() {
         short v = in();
         output() = v > 0 ? v / 2 : v + 11;
     }
@@ -121,7 +121,7 @@ HIPACC_CODEGEN int main(int argc, const char **argv) {
             HipaccAccessor<char> a_blur = hipaccMakeAccessor(pb(-1));
             HipaccAccessor<char> i_down = hipaccMakeAccessor(pg(0));
             Take down;
-            down.in = hipaccDevAccessor(a_blur, HipaccBoundary::UNDEFINED, 0, HIPACC_INTERP_NN, &i_down);
+            down.in = hipaccDevAccessor<true>(a_blur, HipaccBoundary::UNDEFINED, 0, HIPACC_INTERP_NN, &i_down);
             hipaccLaunchUser(down, i_down);
 
             HipaccAccessor<char> a_prev = hipaccMakeAccessor(pg(-1));
@@ -129,7 +129,7 @@ HIPACC_CODEGEN int main(int argc, const char **argv) {
             HipaccAccessor<char> i_lap = hipaccMakeAccessor(pl(-1));
             Detail detail;
             detail.fine = hipaccDevAccessor(a_prev, HipaccBoundary::UNDEFINED, 0);
-            detail.coarse = hipaccDevAccessor(a_up, HipaccBoundary::UNDEFINED, 0, HIPACC_INTERP_LF, &i_lap);
+            detail.coarse = hipaccDevAccessor<true>(a_up, HipaccBoundary::UNDEFINED, 0, HIPACC_INTERP_LF, &i_lap);
             hipaccLaunchUser(detail, i_lap);
         }
 
@@ -140,7 +140,7 @@ HIPACC_CODEGEN int main(int argc, const char **argv) {
             HipaccAccessor<char> a_detail = hipaccMakeAccessor(pl(0));
             HipaccAccessor<char> i_back = hipaccMakeAccessor(pg(0));
             Rebuild rebuild;
-            rebuild.coarse = hipaccDevAccessor(a_coarse, HipaccBoundary::UNDEFINED, 0, HIPACC_INTERP_LF, &i_back);
+            rebuild.coarse = hipaccDevAccessor<true>(a_coarse, HipaccBoundary::UNDEFINED, 0, HIPACC_INTERP_LF, &i_back);
             rebuild.detail = hipaccDevAccessor(a_detail, HipaccBoundary::UNDEFINED, 0);
             hipaccLaunchUser(rebuild, i_back);
         }
@@ -149,13 +149,13 @@ HIPACC_CODEGEN int main(int argc, const char **argv) {
             HipaccAccessor<char> a_g0 = hipaccMakeAccessor(pg(0));
             HipaccAccessor<char> i_restored = hipaccMakeAccessor(restored);
             Take keep_g;
-            keep_g.in = hipaccDevAccessor(a_g0, HipaccBoundary::UNDEFINED, 0);
+            keep_g.in = hipaccDevAccessor<true>(a_g0, HipaccBoundary::UNDEFINED, 0);
             hipaccLaunchUser(keep_g, i_restored);
 
             HipaccAccessor<char> a_l0 = hipaccMakeAccessor(pl(0));
             HipaccAccessor<char> i_lap_out = hipaccMakeAccessor(lap_out);
             Take keep_l;
-            keep_l.in = hipaccDevAccessor(a_l0, HipaccBoundary::UNDEFINED, 0);
+            keep_l.in = hipaccDevAccessor<true>(a_l0, HipaccBoundary::UNDEFINED, 0);
             hipaccLaunchUser(keep_l, i_lap_out);
         }
     });

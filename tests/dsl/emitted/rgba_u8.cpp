@@ -29,7 +29,7 @@ struct Smooth : hipacc_dev::Kernel<uchar4> {
     hipacc_dev::Accessor<uchar4> in;
     hipacc_dev::Mask<float> mask;
     __device__ void set_pos(int x_, int y_) { it_x = x_; it_y = y_; in.px = x_; in.py = y_; }
-    __device__ void kernel() {
+    __device__ __forceinline__ void kernel() {
         float4 s = convolve(mask, Reduce::SUM, [&] () -> float4 {
             return mask() * convert_float4(in(mask));
         });
@@ -42,7 +42,7 @@ struct Shrink : hipacc_dev::Kernel<uchar4> {
     hipacc_dev::Accessor<uchar4> in;
     hipacc_dev::Domain dom;
     __device__ void set_pos(int x_, int y_) { it_x = x_; it_y = y_; in.px = x_; in.py = y_; }
-    __device__ void kernel() {
+    __device__ __forceinline__ void kernel() {
         output() = reduce(dom, Reduce::MIN, [&] () -> uchar4 { return in(dom); });
     }
 };
@@ -51,7 +51,7 @@ struct Shrink : hipacc_dev::Kernel<uchar4> {
 struct Stretch : hipacc_dev::Kernel<uchar4> {
     hipacc_dev::Accessor<uchar4> in;
     __device__ void set_pos(int x_, int y_) { it_x = x_; it_y = y_; in.px = x_; in.py = y_; }
-    __device__ void kernel() {
+    __device__ __forceinline__ void kernel() {
         int4 d = convert_int4(in()) - 128;
         output() = convert_uchar4(max(min(d * 3 + 128, 255), 0));
     }

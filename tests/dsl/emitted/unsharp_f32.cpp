@@ -24,7 +24,7 @@ using namespace hipacc;
 struct Tone : hipacc_dev::Kernel<float> {
     hipacc_dev::Accessor<float> input;
     __device__ void set_pos(int x_, int y_) { it_x = x_; it_y = y_; input.px = x_; input.py = y_; }
-    __device__ void kernel() {
+    __device__ __forceinline__ void kernel() {
         float v = input();
         output() = v * v * 0.5f + v * 0.5f;
     }
@@ -37,7 +37,7 @@ struct Sharpen : hipacc_dev::Kernel<float> {
     hipacc_dev::Accessor<float> Gray;
     hipacc_dev::Accessor<float> InputBlur;
     __device__ void set_pos(int x_, int y_) { it_x = x_; it_y = y_; Gray.px = x_; Gray.py = y_; InputBlur.px = x_; InputBlur.py = y_; }
-    __device__ void kernel() { output() = 2 * Gray() - InputBlur(); }
+    __device__ __forceinline__ void kernel() { output() = 2 * Gray() - InputBlur(); }
 };
 
 // Kernel class Ratio: device code, the kernel() body compiled as written (hipaccLaunchUser)
@@ -46,14 +46,14 @@ struct Ratio : hipacc_dev::Kernel<float> {
     hipacc_dev::Accessor<float> Gray;
     float eps;
     __device__ void set_pos(int x_, int y_) { it_x = x_; it_y = y_; Sharp.px = x_; Sharp.py = y_; Gray.px = x_; Gray.py = y_; }
-    __device__ void kernel() { output() = Sharp() / (Gray() + eps); }
+    __device__ __forceinline__ void kernel() { output() = Sharp() / (Gray() + eps); }
 };
 
 // Kernel class Laplace: device code, the kernel() body compiled as written (hipaccLaunchUser)
 struct Laplace : hipacc_dev::Kernel<float> {
     hipacc_dev::Accessor<float> input;
     __device__ void set_pos(int x_, int y_) { it_x = x_; it_y = y_; input.px = x_; input.py = y_; }
-    __device__ void kernel() {
+    __device__ __forceinline__ void kernel() {
         output() = 4.0f * input() - input(-1, 0) - input(1, 0) - input(0, -1) - input(0, 1)
                    + 0.25f * (float)(x() % 7) - 0.125f * (float)(y() % 5);
     }

@@ -108,7 +108,7 @@ def test_unrecognised_kernel_becomes_device_code(tmp_path):
     p.write_text(src)
     text = emit(str(p), [DSL_HEADERS])
     assert "struct Scale : hipacc_dev::Kernel<float>" in text
-    assert "__device__ void kernel() { output() = sqrtf(input()); }" in text
+    assert "__device__ __forceinline__ void kernel() { output() = sqrtf(input()); }" in text
     assert "hipaccLaunchUser(scale, iscaled);" in text
     assert '#include "hipacc_hip_device.hpp"' in text
     bad = src.replace("    void kernel() { output() = sqrtf(input()); }",
