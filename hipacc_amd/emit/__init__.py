@@ -59,6 +59,7 @@ class KernelClass:
     bin_type: str = ""                    # Kernel<T, TB>: TB
     reduce_text: str = ""                 # reduce(l, r) as written
     binning_text: str = ""                # binning(x, y, pixel) as written
+    bin_reduce: Optional[str] = None      # reduce() of a binning class: SUM MIN MAX PROD
     body_text: str = ""                   # kernel() body as written
     extent: Tuple[int, int] = (0, 0)
 
@@ -245,6 +246,11 @@ class Emitter:
                 k.global_reduce = "USER"
         if k.binning_text and not k.reduce_text:
             raise EmitError(f"kernel class {k.name}: binning() without reduce()")
+        if k.binning_text and reduce_body is not None:
+            try:
+                k.bin_reduce = classify_reduce(reduce_body, reduce_params)
+            except Unsupported:
+                k.bin_reduce = None
         try:
             k.op = classify_kernel(kernel_body, acc, msk, dom)
         except Unsupported as e:
@@ -717,7 +723,9 @@ class Emitter:
             k, _, it = self._binding(name)
             if not k.binning_text:
                 raise EmitError(f"{name}.binned_data(): {k.name} has no binning()")
-            rep = self._run_first(name, f"hipaccBinUser<{k.name}_bins, {k.bin_type}>("
+            red = (f", HIPACC_REDUCE_{k.bin_reduce}" if k.bin_reduce in ("SUM", "MIN", "MAX")
+                   else "")
+            rep = self._run_first(name, f"hipaccBinUser<{k.name}_bins, {k.bin_type}{red}>("
                                         f"{self._functor(name, '_bins')}, {it}, "
                                         f"{m.group(2).strip()})")
             s0 = b + len(text[:m.start()].encode())

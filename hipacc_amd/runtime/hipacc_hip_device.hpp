@@ -106,6 +106,13 @@ template <typename T, bool INTERP = false> struct Accessor {
   __device__ __forceinline__ int y() const { return py; }
 };
 
+// A read of a table the kernel never writes (mask coefficients, domain
+// flags) through the constant address space: at a wave-uniform index (the
+// position convolve / reduce / iterate is at) it becomes a scalar load.
+template <typename T> __device__ __forceinline__ T table_at(const T* p, int i) {
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
 // A domain: enabled positions of a size_x x size_y window; (cx, cy) is the
 // position reduce() / iterate() is at.
 struct Domain {
@@ -128,12 +135,12 @@ template <typename T> struct Mask {
   __device__ __forceinline__ int y() const { return cy - sy / 2; }
   __device__ __forceinline__ int size_x() const { return sx; }
   __device__ __forceinline__ int size_y() const { return sy; }
-  __device__ __forceinline__ T operator()() const { return coef[cy * sx + cx]; }
+  __device__ __forceinline__ T operator()() const { return table_at(coef, cy * sx + cx); }
   __device__ __forceinline__ T operator()(int xf, int yf) const {
-    return coef[(yf + sy / 2) * sx + xf + sx / 2];
+    return table_at(coef, (yf + sy / 2) * sx + xf + sx / 2);
   }
   __device__ __forceinline__ T operator()(const Domain& d) const {
-    return coef[(d.y() + d.sy / 2) * sx + d.x() + d.sx / 2];
+    return table_at(coef, (d.y() + d.sy / 2) * sx + d.x() + d.sx / 2);
   }
 };
 
@@ -171,12 +178,12 @@ template <typename TO> struct Kernel {
   __device__ __forceinline__ auto reduce(Domain& dom, Reduce mode, const F& fun) -> decltype(fun()) {
     int i = 0;
     const int n = dom.sx * dom.sy;
-    while (i < n && !dom.on[i]) ++i;
+    while (i < n && !table_at(dom.on, i)) ++i;
     dom.cy = i / dom.sx;
     dom.cx = i - dom.cy * dom.sx;
     auto result = fun();
     for (++i; i < n; ++i) {
-      if (!dom.on[i]) continue;
+      if (!table_at(dom.on, i)) continue;
       dom.cy = i / dom.sx;
       dom.cx = i - dom.cy * dom.sx;
       result = combine(mode, result, fun());
@@ -185,7 +192,7 @@ template <typename TO> struct Kernel {
   }
   template <typename F> __device__ __forceinline__ void iterate(Domain& dom, const F& fun) {
     for (int i = 0; i < dom.sx * dom.sy; ++i) {
-      if (!dom.on[i]) continue;
+      if (!table_at(dom.on, i)) continue;
       dom.cy = i / dom.sx;
       dom.cx = i - dom.cy * dom.sx;
       fun();
@@ -219,27 +226,40 @@ template <typename TB> struct Binning {
   __device__ __forceinline__ unsigned num_bins() const { return nbins_; }
 };
 
-// Combines v into *p with the user's reduce(), as one atomic update of the
-// 32-bit word (compare-and-swap until no other thread changed it between).
-template <typename F, typename TB>
+// Combines v into the LDS bin *p with the user's reduce(), as one atomic
+// update of the 32-bit word.  RED names the reduce() when it is one of the
+// DSL's + / min / max forms (HIPACC_REDUCE_*; -1 otherwise): then the LDS
+// atomic of that operation does it in one step; any other reduce() is a
+// compare-and-swap loop (until no other lane changed the word in between).
+template <int RED, typename F, typename TB>
 __device__ __forceinline__ void combine_at(const F& k, TB* p, TB v) {
   static_assert(sizeof(TB) == 4, "bins of 32-bit values");
-  unsigned* w = reinterpret_cast<unsigned*>(p);
-  unsigned old = *w, seen;
-  do {
-    seen = old;
-    const TB nv = k.reduce(__builtin_bit_cast(TB, seen), v);
-    old = atomicCAS(w, seen, __builtin_bit_cast(unsigned, nv));
-  } while (old != seen);
+  constexpr bool integral = std::is_integral<TB>::value;
+  if constexpr (RED == HIPACC_REDUCE_SUM && (integral || std::is_same<TB, float>::value)) {
+    atomicAdd(p, v);
+  } else if constexpr (RED == HIPACC_REDUCE_MIN && integral) {
+    atomicMin(p, v);
+  } else if constexpr (RED == HIPACC_REDUCE_MAX && integral) {
+    atomicMax(p, v);
+  } else {
+    unsigned* w = reinterpret_cast<unsigned*>(p);
+    unsigned old = *w, seen;
+    do {
+      seen = old;
+      const TB nv = k.reduce(__builtin_bit_cast(TB, seen), v);
+      old = atomicCAS(w, seen, __builtin_bit_cast(unsigned, nv));
+    } while (old != seen);
+  }
 }
 
-// Bins of the output image: each block combines its pixels' values into
-// bins in LDS, then its bins into the global bins (zero before the launch,
-// as the DSL's binned_data() starts from TB{}).  reduce() must be
-// associative and commutative, as any parallel binning assumes.
-template <typename F, typename T, typename TB>
+// Bins of the output image, 32-bit values: each block combines its pixels'
+// values into bins in LDS (starting from TB{}, as the DSL's binned_data()
+// does) and stores them as one row of `rows`; user_binning_fold then folds
+// the rows bin by bin in block order.  reduce() must be associative and
+// commutative, as any parallel binning assumes.
+template <int RED, typename F, typename T, typename TB>
 __global__ __launch_bounds__(256) void user_binning_kernel(F f, const T* img, int stride, int ox,
-                                                           int oy, int w, int h, TB* bins,
+                                                           int oy, int w, int h, TB* rows,
                                                            unsigned nbins) {
   extern __shared__ __align__(16) unsigned char user_bins_lds[];
   TB* lb = reinterpret_cast<TB*>(user_bins_lds);
@@ -253,11 +273,10 @@ __global__ __launch_bounds__(256) void user_binning_kernel(F f, const T* img, in
     const int y = (int)(i / w), x = (int)(i - (long long)y * w);
     k.bidx_ = 0xffffffffu;
     k.binning((unsigned)x, (unsigned)y, img[(long long)(oy + y) * stride + ox + x]);
-    if (k.bidx_ < nbins) combine_at(k, lb + k.bidx_, k.bval_);
+    if (k.bidx_ < nbins) combine_at<RED>(k, lb + k.bidx_, k.bval_);
   }
   __syncthreads();
-  for (unsigned b = threadIdx.x; b < nbins; b += 256)
-    if (lb[b] != TB{}) combine_at(k, bins + b, lb[b]);
+  for (unsigned b = threadIdx.x; b < nbins; b += 256) rows[(long long)blockIdx.x * nbins + b] = lb[b];
 }
 
 // Bins of values wider than 32 bits (no atomic update of such a value):
@@ -471,7 +490,9 @@ int hipaccLaunchUser(const F& f, const HipaccAccessor<TO>& is,
 
 // binned_data(num_bins) of a Kernel<T, TB> class over the image its
 // iteration space wrote: a new TB[num_bins] the caller owns.  Blocking.
-template <typename F, typename TB, typename T>
+// RED: HIPACC_REDUCE_SUM / MIN / MAX when reduce() is that operation (one
+// LDS atomic per pixel), -1 for any other reduce() (compare-and-swap).
+template <typename F, typename TB, int RED = -1, typename T>
 TB* hipaccBinUser(const F& f, const HipaccAccessor<T>& is, unsigned num_bins,
                   const HipaccExecutionParameterHip& ep = nullptr) {
   TB* result = new TB[num_bins]();
@@ -487,29 +508,28 @@ TB* hipaccBinUser(const F& f, const HipaccAccessor<T>& is, unsigned num_bins,
   const long long n = (long long)v.width * v.height;
   const int blocks = (int)std::min<long long>(std::max<long long>((n + 4095) / 4096, 1),
                                               narrow ? 1024 : 512);
-  // the bins, and for wide values the per-block rows after them
-  const size_t words = (lds + 3) / 4 + (narrow ? 0 : (size_t)blocks * lds / 4);
+  // the bins, then one row of bins per block
+  const size_t words = (lds + 3) / 4 + (size_t)blocks * lds / 4;
   void* bins = nullptr;
   int32_t stride = 0;
   if (hipaccCheck(hipacc_hip_alloc2d(&bins, &stride, (int32_t)words, 1, 4, 0), "hipaccBinUser"))
     return result;
   hipStream_t s = static_cast<hipStream_t>(hipaccStreamOf(ep));
-  if (narrow) hipaccCheck(hipacc_hip_fill(bins, 0u, num_bins, 4, s), "hipaccBinUser");
   if (ep) ep->pre_kernel();
   HipaccLaunchTimer timer(s);
   TB* out = static_cast<TB*>(bins);
+  TB* rows = reinterpret_cast<TB*>(static_cast<char*>(bins) + (lds + 3) / 4 * 4);
   if constexpr (narrow) {
-    hipLaunchKernelGGL((hipacc_dev::user_binning_kernel<F, T, TB>), dim3(blocks), dim3(256), lds,
-                       s, f, static_cast<const T*>(v.data), v.stride, v.offset_x, v.offset_y,
-                       v.width, v.height, out, num_bins);
+    hipLaunchKernelGGL((hipacc_dev::user_binning_kernel<RED, F, T, TB>), dim3(blocks), dim3(256),
+                       lds, s, f, static_cast<const T*>(v.data), v.stride, v.offset_x,
+                       v.offset_y, v.width, v.height, rows, num_bins);
   } else {
-    TB* rows = reinterpret_cast<TB*>(static_cast<char*>(bins) + (lds + 3) / 4 * 4);
     hipLaunchKernelGGL((hipacc_dev::user_binning_wide_kernel<F, T, TB>), dim3(blocks), dim3(64),
                        lds, s, f, static_cast<const T*>(v.data), v.stride, v.offset_x,
                        v.offset_y, v.width, v.height, rows, num_bins);
-    hipLaunchKernelGGL((hipacc_dev::user_binning_fold<F, TB>), dim3((num_bins + 255) / 256),
-                       dim3(256), 0, s, f, rows, blocks, num_bins, out);
   }
+  hipLaunchKernelGGL((hipacc_dev::user_binning_fold<F, TB>), dim3((num_bins + 255) / 256),
+                     dim3(256), 0, s, f, rows, blocks, num_bins, out);
   const hipError_t e = hipGetLastError();
   const int st = hipaccCheck(e == hipSuccess ? HIPACC_OK : HIPACC_ERR_HIP, "hipaccBinUser");
   timer.finish(st);

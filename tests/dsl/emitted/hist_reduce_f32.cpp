@@ -60,7 +60,7 @@ HIPACC_CODEGEN int main(int argc, const char **argv) {
     HipaccAccessor<float> icopy = hipaccMakeAccessor(copy);
     // Histogram hist(icopy, acc): launched at hist.execute()
     hipaccLaunchPoint(HIPACC_POINT_COPY, acc, icopy);
-    uint *bins = hipaccBinUser<Histogram_bins, uint>(Histogram_bins{}, icopy, num_bins);
+    uint *bins = hipaccBinUser<Histogram_bins, uint, HIPACC_REDUCE_SUM>(Histogram_bins{}, icopy, num_bins);
 
     HipaccAccessor<float> ishifted = hipaccMakeAccessor(shifted);
     // MaxMagnitude maxmag(ishifted, acc): launched at maxmag.execute()

@@ -131,7 +131,7 @@ HIPACC_CODEGEN int main(int argc, const char **argv) {
 
     HipaccAccessor<uchar4> i_copy = hipaccMakeAccessor(copy);
     // TileSums k_tiles(i_copy, a_in, tiles_x): launched at k_tiles.execute()
-    float4 *bins = [&] { hipaccLaunchPoint(HIPACC_POINT_COPY, a_in, i_copy); return hipaccBinUser<TileSums_bins, float4>([&] { TileSums_bins f; f.tiles_x = tiles_x; return f; }(), i_copy, tiles_x * tiles_y); }();
+    float4 *bins = [&] { hipaccLaunchPoint(HIPACC_POINT_COPY, a_in, i_copy); return hipaccBinUser<TileSums_bins, float4, HIPACC_REDUCE_SUM>([&] { TileSums_bins f; f.tiles_x = tiles_x; return f; }(), i_copy, tiles_x * tiles_y); }();
 
     FILE *f = std::fopen(argv[1], "wb");
     std::fwrite(hipaccReadMemory<uchar4>(smooth), sizeof(uchar4), width * height, f);
