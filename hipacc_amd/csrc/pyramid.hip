@@ -152,8 +152,12 @@ __global__ __launch_bounds__(256) void pyramid_level_kernel(const PyrLevelArgs a
           gls[r * RW + c] = pyr_read(G, gs, W, H, a.boundary, rx0 + c, ry0 + r);
     }
     __syncthreads();
+    // i / uw as a multiply by the rounded-up reciprocal ceil(2^32 / uw):
+    // exact for i < 2^32 / uw (i < UW * UH here); uw == 1 has no 32-bit
+    // reciprocal and takes i itself
+    const unsigned recip = uw > 1 ? 0xffffffffu / (unsigned)uw + 1u : 0u;
     for (int i = threadIdx.x; i < uw * uh; i += 256) {
-      const int yy = i / uw, xx = i - yy * uw;
+      const int yy = uw > 1 ? (int)__umulhi((unsigned)i, recip) : i, xx = i - yy * uw;
       const int xu = ux0 + xx, yu = uy0 + yy;
       const int nx = (int)interp_map(sd, xu) - 2 - rx0, ny = (int)interp_map(td, yu) - 2 - ry0;
       const float* src = gls + ny * RW + nx;
@@ -176,27 +180,36 @@ __global__ __launch_bounds__(256) void pyramid_level_kernel(const PyrLevelArgs a
     __syncthreads();
 
     // a lane keeps the columns lane, lane+64, ... (their LF column weights
-    // are computed once) and walks the rows wave, wave+4, ...
+    // are computed once) and walks the rows wave, wave+4, ... (the row's LF
+    // weights computed once for all its columns)
+    constexpr int NCX = TW / 64;
+    int xa[NCX], xb[NCX];
+    float fx[NCX];
 #pragma unroll
-    for (int cx = 0; cx < TW / 64; ++cx) {
+    for (int cx = 0; cx < NCX; ++cx) {
       const int x = x0 + lane + 64 * cx;
-      if (x < W) {
-        int xi;
-        float fx;
-        interp_lf(interp_map(su, x), xi, fx);
-        const int xa = xi - ux0, xb = min(xi + 1, W1 - 1) - ux0;
-        for (int r = wave; r < TH; r += 4) {
-          const int y = y0 + r;
-          if (y >= H) break;
-          int yi;
-          float fy;
-          interp_lf(interp_map(sv, y), yi, fy);
-          const int ya = yi - uy0, yb = min(yi + 1, H1 - 1) - uy0;
-          const float lf = interp_bilinear(fx, fy, g1s[ya * UW + xa], g1s[ya * UW + xb],
-                                           g1s[yb * UW + xa], g1s[yb * UW + xb]);
+      int xi;
+      interp_lf(interp_map(su, x), xi, fx[cx]);
+      xa[cx] = xi - ux0;
+      xb[cx] = min(xi + 1, W1 - 1) - ux0;
+    }
+    const int rows = min(TH, H - y0);
+    for (int r = wave; r < rows; r += 4) {
+      const int y = y0 + r;
+      int yi;
+      float fy;
+      interp_lf(interp_map(sv, y), yi, fy);
+      const int ya = (yi - uy0) * UW, yb = (min(yi + 1, H1 - 1) - uy0) * UW;
+      const int grow = (y - ry0) * RW - rx0;
+      float* lrow = L + (long long)y * ls;
+#pragma unroll
+      for (int cx = 0; cx < NCX; ++cx) {
+        const int x = x0 + lane + 64 * cx;
+        if (x < W) {
+          const float lf = interp_bilinear(fx[cx], fy, g1s[ya + xa[cx]], g1s[ya + xb[cx]],
+                                           g1s[yb + xa[cx]], g1s[yb + xb[cx]]);
           // L[l] is not read again here: non-temporal stores
-          __builtin_nontemporal_store(gls[(y - ry0) * RW + (x - rx0)] - lf,
-                                      L + (long long)y * ls + x);
+          __builtin_nontemporal_store(gls[grow + x] - lf, lrow + x);
         }
       }
     }
