@@ -364,21 +364,25 @@ class Emitter:
             self._visit(ch, ch.extent() if ch.kind == "CompoundStmt" else scope)
 
     def _statement(self, st: C.Cursor) -> None:
-        toks = st.tokens()
+        """A declaration statement, one or more declarators of one type."""
         b, e = st.extent()
         if self.src[e:e + 1] == b";":
             e += 1
-        decls = [ch for ch in st.children() if ch.kind == "VarDecl"]
-        if len(decls) != 1:
+        reps = []
+        for v in (ch for ch in st.children() if ch.kind == "VarDecl"):
+            toks = v.tokens()
+            name, ty = v.spelling, v.type
+            kind, targ = _tpl(ty)
+            i = toks.index(name) if name in toks else -1
+            args = (_args(toks, i + 1)[0] if 0 <= i < len(toks) - 1 and toks[i + 1] == "("
+                    else [])
+            reps.append((name, self._declaration(name, kind, targ, args)))
+        if all(r is None for _, r in reps):
             return
-        v = decls[0]
-        name, ty = v.spelling, v.type
-        kind, targ = _tpl(ty)
-        i = toks.index(name) if name in toks else -1
-        args = _args(toks, i + 1)[0] if 0 <= i < len(toks) - 1 and toks[i + 1] == "(" else []
-        rep = self._declaration(name, kind, targ, args)
-        if rep is not None:
-            self.edits.append((b, e, rep))
+        if any(r is None for _, r in reps):
+            raise EmitError("a declaration mixes DSL objects and other variables: "
+                            + ", ".join(n for n, _ in reps))
+        self.edits.append((b, e, "\n    ".join(r for _, r in reps)))
 
     def _call(self, st: C.Cursor, scope: Tuple[int, int]) -> bool:
         """A call statement the emitter rewrites; False to look inside it."""
