@@ -30,8 +30,8 @@ from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence, Tuple
 
 from . import clang as C
-from .recognise import (GlobalOp, LocalOp, PointOp, Unsupported, classify_kernel,
-                        classify_reduce)
+from .recognise import (BilateralOp, GlobalOp, LocalOp, PointOp, Unsupported,
+                        classify_bilateral, classify_kernel, classify_reduce)
 
 
 class EmitError(RuntimeError):
@@ -255,6 +255,17 @@ class Emitter:
             k.op = classify_kernel(kernel_body, acc, msk, dom)
         except Unsupported as e:
             k.op = UserOp(kernel_text, str(e))
+            scalars = [n for n, (kd, _) in k.fields.items()
+                       if kd not in ("Accessor", "Mask", "Domain")]
+            try:
+                op = classify_bilateral(kernel_body, acc, msk, dom, scalars)
+                pix = k.fields[op.accessor][1]
+                if (op.round and pix in ("uchar", "unsigned char") and
+                        k.out_type in ("uchar", "unsigned char")) or \
+                        (not op.round and pix == "float" and k.out_type == "float"):
+                    k.op = op
+            except Unsupported:
+                pass
         return k
 
     # ---- main() -----------------------------------------------------------------------
@@ -544,6 +555,15 @@ class Emitter:
         op = k.op
         if isinstance(op, UserOp):
             return f"hipaccLaunchUser({inst}, {it});"
+        if isinstance(op, BilateralOp):
+            a = self._acc(inst, bind[op.accessor])
+            sp = f"{inst}_spatial"
+            return ("{\n        "
+                    f"auto {sp} = hipaccBilateralSpatial({bind[op.mask]}, {bind[op.domain]});\n"
+                    f"        hipaccLaunchBilateral({bind[op.accessor]}, {it}, "
+                    f"{bind[op.mask]}.size_x, 0.0f, (float)({bind[op.sigma_r]}), {sp}.data(), "
+                    f"HipaccBoundary::{a.mode}, {a.value}, {'true' if op.round else 'false'});"
+                    "\n    }")
         if isinstance(op, PointOp):
             ins = [bind[a] for a in op.inputs]
             if op.op in ("COPY", "AFFINE"):
@@ -640,6 +660,8 @@ class Emitter:
             return None
         if isinstance(k.op, LocalOp):
             return bind[k.op.accessor]
+        if not isinstance(k.op, PointOp):
+            return None
         if k.op.op in ("COPY", "AFFINE", "ADD", "SUB", "MUL"):
             return bind[k.op.inputs[0]]
         return None
@@ -832,6 +854,9 @@ def _describe_kernel(k: KernelClass) -> str:
         return "device code, the kernel() body compiled as written (hipaccLaunchUser)"
     if isinstance(op, PointOp):
         return f"point operator {op.op} (hipaccLaunchPoint)"
+    if isinstance(op, BilateralOp):
+        return (f"bilateral filter over domain {op.domain}, spatial weights {op.mask}"
+                f"{', rounded' if op.round else ''} (hipaccLaunchBilateral)")
     return (f"local operator {op.reduce} over {'mask ' + op.mask if op.mask else 'domain ' + op.domain}"
             f"{', rounded' if op.round else ''} (hipaccLaunchLocal)")
 

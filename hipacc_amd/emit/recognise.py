@@ -15,6 +15,9 @@ lib/AST/Convolution.cpp:228-467, dsl/kernel.hpp:241-296):
           a local variable initialised from the input and updated with
           `v += c` / `v -= c` / `v *= c` before `output() = v`
   global  reduce(l, r) { return min(l, r) | max(l, r) | l + r | l * r; }
+  bilateral  the Bilateral_Filter sample's kernel (samples-public/3_Preprocessing/
+          Bilateral_Filter/src/main.cpp:63-77), statement for statement, with
+          any names, rounded to an integer output or written as a float
 
 Anything else raises Unsupported with the reason: the emitter never guesses
 an operator whose arithmetic it cannot reproduce exactly.
@@ -193,10 +196,67 @@ class PointOp:
 
 
 @dataclass
+class BilateralOp:
+    accessor: str
+    mask: str                    # spatial weights (a Mask member)
+    domain: str                  # the positions iterated (a Domain member)
+    sigma_r: str                 # the scalar member holding sigma_r
+    round: bool                  # (T)(p / d + 0.5f) instead of p / d
+    kind: str = "bilateral"
+
+
+@dataclass
 class GlobalOp:
     reduce: str                  # SUM MIN MAX PROD
     accessor: str = ""
     kind: str = "global"
+
+
+# The Bilateral_Filter sample's kernel() as tokens; $NAME binds one
+# identifier token (the same token wherever the name recurs), $T the output
+# type.  {CAST} is "( float )" or nothing, {OUT} the rounded or the float
+# output statement.
+_BILATERAL = (
+    "{ float $c = 0.5f / ( $S * $S ) ; float $d = 0.0f ; float $p = 0.0f ; "
+    "float $ctr = $IN ( ) ; "
+    "iterate ( $DOM , [ & ] ( ) -> void { "
+    "float $diff = {CAST} ( $IN ( $DOM ) ) - $ctr ; "
+    "float $s = expf ( - $c * $diff * $diff ) * $MASK ( $DOM ) ; "
+    "$d += $s ; $p += $s * $IN ( $DOM ) ; } ) ; "
+    "{OUT} }")
+_BILATERAL_OUT = {True: "output ( ) = ( $T ) ( $p / $d + 0.5f ) ;",
+                  False: "output ( ) = $p / $d ;"}
+
+
+def _match(template: List[str], toks: List[str]) -> Optional[Dict[str, str]]:
+    if len(template) != len(toks):
+        return None
+    env: Dict[str, str] = {}
+    for t, k in zip(template, toks):
+        if t.startswith("$"):
+            if not re.match(r"^[A-Za-z_]\w*$", k) or env.setdefault(t, k) != k:
+                return None
+        elif t != k:
+            return None
+    return env
+
+
+def classify_bilateral(body: List[str], accessors: List[str], masks: List[str],
+                       domains: List[str], scalars: List[str]) -> BilateralOp:
+    """The bilateral filter, if body is the Bilateral_Filter sample's kernel."""
+    for rnd in (True, False):
+        for cast in ("( float )", ""):
+            tpl = _BILATERAL.replace("{CAST}", cast).replace("{OUT}", _BILATERAL_OUT[rnd]).split()
+            env = _match(tpl, body)
+            if env is None:
+                continue
+            names = [env[k] for k in ("$c", "$d", "$p", "$ctr", "$diff", "$s")]
+            if (env["$IN"] in accessors and env["$DOM"] in domains and env["$MASK"] in masks
+                    and env["$S"] in scalars and len(set(names)) == len(names)
+                    and not set(names) & {env["$IN"], env["$DOM"], env["$MASK"], env["$S"]}
+                    and env.get("$T", "uchar") in ("uchar", "unsigned char")):
+                return BilateralOp(env["$IN"], env["$MASK"], env["$DOM"], env["$S"], rnd)
+    raise Unsupported("not the bilateral filter's form")
 
 
 def _split_statements(toks: List[str]) -> List[List[str]]:

@@ -297,6 +297,24 @@ int hipaccLaunchFused(const HipaccAccessor<TI>& in, const HipaccAccessor<TO>& ou
 
 // The bilateral filter kernel (range weight from sigma_r, spatial weight
 // from sigma_d or a host table).
+// The spatial table of a bilateral kernel written in the DSL (a Mask of
+// weights iterated over a Domain): the mask's coefficients, 0 where the
+// domain has a hole (a tap of weight 0 adds exactly 0 to both sums, as a
+// skipped tap does).  Empty if the two are not one square window.
+template <typename M, typename D>
+std::vector<float> hipaccBilateralSpatial(const HipaccMask<M>& mask, const HipaccMask<D>& dom) {
+  std::vector<float> w;
+  if (mask.size_x != mask.size_y || dom.size_x != mask.size_x || dom.size_y != mask.size_y) {
+    hipaccRuntimeLogTrivial(hipaccRuntimeLogLevel::ERROR,
+                            "ERROR: hipaccBilateralSpatial: mask and domain are not one square "
+                            "window");
+    return w;
+  }
+  w.resize(mask.coef.size());
+  for (size_t i = 0; i < w.size(); ++i) w[i] = dom.domain[i] ? (float)mask.coef[i] : 0.0f;
+  return w;
+}
+
 template <typename T>
 int hipaccLaunchBilateral(const HipaccAccessor<T>& in, const HipaccAccessor<T>& out, int ksize,
                           float sigma_d, float sigma_r, const float* spatial = nullptr,

@@ -3,7 +3,7 @@
 // closeness mask computed on the host, a Domain over it, and iterate() with
 // the range weight expf(-c_r * diff^2) per tap (CLAMP).  Writes the output
 // image to argv[1].
-#include "hipacc_hip_device.hpp"
+#include "hipacc_hip.hpp"
 // DSL names kept by the emitted host code
 typedef unsigned char uchar;
 typedef unsigned short ushort;
@@ -12,7 +12,6 @@ typedef unsigned int uint;
 #define HIPACC_CODEGEN
 #endif
 namespace hipacc { namespace math { using std::min; using std::max; } }
-namespace hipacc { using Reduce = hipacc_dev::Reduce; }
 
 #include <cmath>
 #include <cstdint>
@@ -20,29 +19,7 @@ namespace hipacc { using Reduce = hipacc_dev::Reduce; }
 
 using namespace hipacc;
 
-// Kernel class BilateralFilter: device code, the kernel() body compiled as written (hipaccLaunchUser)
-struct BilateralFilter : hipacc_dev::Kernel<uchar> {
-    hipacc_dev::Accessor<uchar> in;
-    hipacc_dev::Mask<float> mask;
-    hipacc_dev::Domain dom;
-    int sigma_r;
-    __device__ void set_pos(int x_, int y_) { it_x = x_; it_y = y_; in.px = x_; in.py = y_; }
-    __device__ __forceinline__ void kernel() {
-        float c_r = 0.5f / (sigma_r * sigma_r);
-        float d = 0.0f;
-        float p = 0.0f;
-        float center = in();
-
-        iterate(dom, [&] () -> void {
-            float diff = (float)(in(dom)) - center;
-            float s = expf(-c_r * diff * diff) * mask(dom);
-            d += s;
-            p += s * in(dom);
-        });
-
-        output() = (uchar)(p / d + 0.5f);
-    }
-};
+// Kernel class BilateralFilter: bilateral filter over domain dom, spatial weights mask, rounded (hipaccLaunchBilateral)
 
 static uint32_t pixel_hash(int x, int y) {
     uint32_t h = (uint32_t)(x / 3) * 73856093u ^ (uint32_t)(y / 3) * 19349663u;
@@ -74,12 +51,11 @@ HIPACC_CODEGEN int main(int argc, const char **argv) {
     // BoundaryCondition bound: CLAMP of in
     HipaccAccessor<uchar> acc = hipaccMakeAccessor(in);
     HipaccAccessor<uchar> iter = hipaccMakeAccessor(out);
-    BilateralFilter filter;
-    filter.in = hipaccDevAccessor(acc, HipaccBoundary::CLAMP, 0);
-    filter.mask = hipaccDevMask(mask);
-    filter.dom = hipaccDevDomain(dom);
-    filter.sigma_r = sigma_r;
-    hipaccLaunchUser(filter, iter);
+    // BilateralFilter filter(iter, acc, mask, dom, sigma_r): launched at filter.execute()
+    {
+        auto filter_spatial = hipaccBilateralSpatial(mask, dom);
+        hipaccLaunchBilateral(acc, iter, mask.size_x, 0.0f, (float)(sigma_r), filter_spatial.data(), HipaccBoundary::CLAMP, 0, true);
+    }
 
     FILE *f = std::fopen(argv[1], "wb");
     std::fwrite(hipaccReadMemory<uchar>(out), 1, (size_t)width * height, f);

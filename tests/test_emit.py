@@ -162,5 +162,8 @@ def test_emitter_takes_every_reference_sample(name):
     if name == "Motion_Interpolation":
         assert "hipaccWriteMemory<float4>(merged_vec, vecs);" in text
         assert "[&] { hipaccLaunchPoint(HIPACC_POINT_COPY, acc_vec, iter_vec);" in text
+    if name == "Bilateral_Filter":                # the library's bilateral kernel
+        assert "hipaccLaunchBilateral(acc, iter, mask.size_x" in text
+        assert "hipaccLaunchUser" not in text
     if name.startswith("Kernel_Fusion_"):        # the whole chain in one launch
         assert text.count("hipaccLaunchFused(") == 1 and "hipaccLaunchUser" not in text
