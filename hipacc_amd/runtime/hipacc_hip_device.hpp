@@ -488,6 +488,105 @@ int hipaccLaunchUser(const F& f, const HipaccAccessor<TO>& is,
   return st;
 }
 
+// ---------------------------------------------------------------------------
+// Launching a kernel of the program's own in the reference's generated-code
+// form (runtime/hipacc_base.hpp:87-101, runtime/hipacc_base_standalone.hpp:
+// 41-58, runtime/hipacc_cu_standalone.hpp:66-110,291-329): the launch
+// geometry of an iteration space, the grid for a block shape, the block
+// indices between which a block needs no border handling, and a launch on an
+// execution parameter's stream, timed when timing is on.
+// ---------------------------------------------------------------------------
+struct hipacc_launch_info {
+  // size_x / size_y: half the largest window read with boundary handling
+  // (0 for none); the iteration space's extent and offset; pixels per
+  // thread (rows) and SIMD width (columns).
+  hipacc_launch_info(int size_x_, int size_y_, int is_width_, int is_height_, int offset_x_,
+                     int offset_y_, int pixels_per_thread_, int simd_width_)
+      : size_x(size_x_), size_y(size_y_), is_width(is_width_), is_height(is_height_),
+        offset_x(offset_x_), offset_y(offset_y_), pixels_per_thread(pixels_per_thread_),
+        simd_width(simd_width_) {}
+  hipacc_launch_info(int size_x_, int size_y_, const HipaccAccessorBase& is,
+                     int pixels_per_thread_, int simd_width_)
+      : hipacc_launch_info(size_x_, size_y_, is.width, is.height, is.offset_x, is.offset_y,
+                           pixels_per_thread_, simd_width_) {}
+  int size_x, size_y;
+  int is_width, is_height;
+  int offset_x, offset_y;
+  int pixels_per_thread, simd_width;
+  // set by hipaccPrepareKernelLaunch
+  int bh_start_left = 0, bh_start_right = 0;
+  int bh_start_top = 0, bh_start_bottom = 0;
+  int bh_fall_back = 0;
+};
+
+// Blocks covering the iteration space: block.x * simd_width columns and
+// block.y * pixels_per_thread rows each (the columns from 0, the offset
+// included, as the generated kernels index them).
+inline dim3 hipaccCalcGridFromBlock(const hipacc_launch_info& info, const dim3& block) {
+  const int bw = (int)block.x * info.simd_width, bh = (int)block.y * info.pixels_per_thread;
+  return dim3((unsigned)((info.is_width + info.offset_x + bw - 1) / bw),
+              (unsigned)((info.is_height + bh - 1) / bh));
+}
+
+// Block columns [bh_start_left, bh_start_right) and rows [bh_start_top,
+// bh_start_bottom) need no border handling: their pixels' windows stay
+// inside the image.  The left bound is the first block whose first column
+// is past offset_x + size_x; the right bound the blocks that end before the
+// last size_x columns; the bottom bound keeps 2 * size_y rows of blocks
+// back, room for the rows a shared-memory variant stages below a block.  With
+// fewer than two interior blocks either way every block takes the border
+// path (bh_fall_back).
+inline void hipaccPrepareKernelLaunch(hipacc_launch_info& info, const dim3& block) {
+  const int bw = (int)block.x * info.simd_width;
+  const int bh = (int)block.y * info.pixels_per_thread;
+  auto div_up = [](int a, int b) { return a <= 0 ? -((-a) / b) : (a + b - 1) / b; };
+  auto div_down = [](int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); };
+  if (info.size_x > 0) {
+    info.bh_start_left = div_up(info.offset_x + info.size_x, bw);
+    info.bh_start_right = div_down(info.offset_x + info.is_width - info.size_x, bw);
+  } else {
+    info.bh_start_left = 0;
+    info.bh_start_right = div_down(info.offset_x + info.is_width, bw);
+  }
+  if (info.size_y > 0) {
+    const int staged = div_up(2 * info.size_y, (int)block.y);
+    info.bh_start_top = div_up(info.size_y, bh);
+    info.bh_start_bottom = div_down(info.is_height - staged * (int)block.y, bh);
+  } else {
+    info.bh_start_top = 0;
+    info.bh_start_bottom = div_down(info.is_height, bh);
+  }
+  info.bh_fall_back = (info.bh_start_right - info.bh_start_left) > 1 &&
+                              (info.bh_start_bottom - info.bh_start_top) > 1
+                          ? 0
+                          : 1;
+}
+
+// hipaccLaunchKernel(kernel, grid, block, ep, print_timing, shared_memory,
+// args...): the launch of a __global__ function on the execution
+// parameter's stream; with print_timing (or the calling thread's timing
+// switch on) the launch is timed for hipacc_last_kernel_timing().
+template <typename... P, typename... A>
+int hipaccLaunchKernel(void (*kernel)(P...), const dim3& grid, const dim3& block,
+                       const HipaccExecutionParameterHip& ep, bool print_timing,
+                       size_t shared_memory, A&&... args) {
+  hipStream_t s = static_cast<hipStream_t>(hipaccStreamOf(ep));
+  const int was = hipacc_hip_get_timing();
+  if (print_timing) hipacc_hip_set_timing(1);
+  if (ep) ep->pre_kernel();
+  int st;
+  {
+    HipaccLaunchTimer timer(s);
+    hipLaunchKernelGGL(kernel, grid, block, (unsigned)shared_memory, s, std::forward<A>(args)...);
+    const hipError_t e = hipGetLastError();
+    st = hipaccCheck(e == hipSuccess ? HIPACC_OK : HIPACC_ERR_HIP, "hipaccLaunchKernel");
+    timer.finish(st);
+  }
+  if (ep) ep->post_kernel();
+  if (print_timing) hipacc_hip_set_timing(was);
+  return st;
+}
+
 // binned_data(num_bins) of a Kernel<T, TB> class over the image its
 // iteration space wrote: a new TB[num_bins] the caller owns.  Blocking.
 // RED: HIPACC_REDUCE_SUM / MIN / MAX when reduce() is that operation (one

@@ -129,3 +129,19 @@ def test_runtime_kernel_timing(results):
     t_on, t_off = (float(v) for v in scal["timing"])
     assert t_on > 0.0
     assert t_off == t_on
+
+
+LAUNCH = ROOT / "tests" / "cpp" / "launch_sample"
+
+
+def test_own_kernel_launched_through_the_runtime(gpu, tmp_path):
+    """hipacc_launch_info / hipaccCalcGridFromBlock / hipaccPrepareKernelLaunch
+    / hipaccLaunchKernel (runtime/hipacc_base.hpp:87-101,
+    runtime/hipacc_cu_standalone.hpp:66-110,291-329) with a kernel of the
+    program's own: outputs equal the host's, no block inside the border
+    bounds reads outside the image, the timed launch reports a time."""
+    assert LAUNCH.exists(), f"{LAUNCH} is not built (make -C tests/cpp)"
+    print("launch_sample", flush=True)
+    p = subprocess.run([str(LAUNCH)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(p.stdout)
+    assert p.returncode == 0 and "PASSED" in p.stdout, p.stdout + p.stderr
