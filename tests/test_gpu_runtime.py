@@ -134,6 +134,7 @@ def test_runtime_kernel_timing(results):
 LAUNCH = ROOT / "tests" / "cpp" / "launch_sample"
 
 
+@pytest.mark.gpu
 def test_own_kernel_launched_through_the_runtime(gpu, tmp_path):
     """hipacc_launch_info / hipaccCalcGridFromBlock / hipaccPrepareKernelLaunch
     / hipaccLaunchKernel (runtime/hipacc_base.hpp:87-101,
