@@ -1204,6 +1204,10 @@ static int run_local_grad(const hipacc_local_grad_desc* d, int variant, char* ws
   a.partials = reinterpret_cast<float*>(ws + lay.part);
   a.dw_variant = variant;
   Timer tm(s, timed);
+  // dw's partials first: their reduction then rides on the fold's launch
+  PartialSum sum{};
+  if (dw && (st = launch_conv_dw(a, d->grad_weights_dev, s, &sum))) return tm.finish(st);
+  bool summed = !dw;
   if (dx) {
     // E over the image: the forward operator on dY with the flipped mask and
     // a zero border.
@@ -1214,9 +1218,17 @@ static int run_local_grad(const hipacc_local_grad_desc* d, int variant, char* ws
     f.mask_w = kx;
     f.mask_h = ky;
     float flipped[kMaxTaps];
-    if (d->weights_dev) {
-      float* wf = reinterpret_cast<float*>(ws + lay.flip);
+    const bool folds = d->boundary != HIPACC_BOUNDARY_CONSTANT;
+    float* ring = reinterpret_cast<float*>(ws + lay.ring);
+    float* wf = d->weights_dev ? reinterpret_cast<float*>(ws + lay.flip) : nullptr;
+    // the ring pass (which also writes the flipped device mask) before the
+    // image pass that reads that mask
+    if (folds) {
+      if ((st = launch_conv_ring(a, ring, wf, s))) return tm.finish(st);
+    } else if (wf) {
       if ((st = launch_flip(d->weights_dev, wf, kk, s))) return tm.finish(st);
+    }
+    if (d->weights_dev) {
       f.weights_dev = wf;
     } else {
       for (int t = 0; t < kk; ++t) flipped[t] = d->weights_f32[kk - 1 - t];
@@ -1235,15 +1247,14 @@ static int run_local_grad(const hipacc_local_grad_desc* d, int variant, char* ws
     bool box = false;
     if ((st = prep_local(&f, la, accf, box, sa))) return tm.finish(st);
     if ((st = launch_local(la, accf, s, kVarAuto, 0))) return tm.finish(st);
-    if (d->boundary != HIPACC_BOUNDARY_CONSTANT) {
-      float* ring = reinterpret_cast<float*>(ws + lay.ring);
-      if ((st = launch_conv_ring(a, ring, s))) return tm.finish(st);
+    if (folds) {
       if ((st = launch_fold_add(a.gx, a.gx_plane, a.gx_stride, a.w, a.h, a.batch, kx / 2, ky / 2,
-                                d->boundary, ring, s)))
+                                d->boundary, ring, s, dw ? &sum : nullptr)))
         return tm.finish(st);
+      summed = true;
     }
   }
-  if (dw) st = launch_conv_dw(a, d->grad_weights_dev, s);
+  if (!summed) st = launch_partial_sum(sum, s);
   return tm.finish(st);
 }
 

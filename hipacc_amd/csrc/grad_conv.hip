@@ -35,52 +35,153 @@ int launch_flip(const float* w, float* out, int n, hipStream_t s) {
   return hip_status(hipGetLastError(), "flip_kernel launch");
 }
 
-// One thread per ring position of one image.
+// One thread per ring position of one image (KX, KY: the mask size when it
+// is a compile-time constant, unrolled with the taps outside the image
+// masked; 0: any size).  Block (0, 0) also writes the flipped mask.
+template <int KX, int KY, typename I>
 __global__ __launch_bounds__(256) void conv_ring_kernel(const ConvGradArgs a, float* ring,
-                                                        long long rsize) {
-  const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
+                                                        long long rsize, float* wflip) {
+  const int kx = KX ? KX : a.kx, ky = KY ? KY : a.ky;
   const int b = blockIdx.y;
-  if (j >= rsize) return;
-  const int rx = a.kx / 2, ry = a.ky / 2;
+  if (wflip && b == 0 && blockIdx.x == 0)
+    for (int t = threadIdx.x; t < kx * ky; t += 256) wflip[t] = grad_weight(a, kx * ky - 1 - t);
+  const I j = (I)blockIdx.x * 256 + threadIdx.x;
+  if (j >= (I)rsize) return;
+  const int rx = kx / 2, ry = ky / 2;
   int ix, iy;
-  ring_pos(a.w, a.h, rx, ry, j, ix, iy);
+  ring_pos<I>(a.w, a.h, rx, ry, j, ix, iy);
   const float* G = a.gy + b * a.gy_plane;
   float e = 0.0f;
-  for (int ty = 0; ty < a.ky; ++ty) {
-    const int y = iy + ry - ty;
-    if (y < 0 || y >= a.h) continue;
-    for (int tx = 0; tx < a.kx; ++tx) {
-      const int x = ix + rx - tx;
-      if (x < 0 || x >= a.w) continue;
-      e = __builtin_fmaf(grad_weight(a, ty * a.kx + tx), G[(long long)y * a.gy_stride + x], e);
+  if constexpr (KX > 0 && KY > 0) {
+    // the loads of a tap row issued together (rows outside the image, the
+    // same for the lanes of a wave in the top and bottom bands, skipped);
+    // columns clamped into the image, the taps outside it leave e unchanged
+#pragma unroll
+    for (int ty = 0; ty < KY; ++ty) {
+      const int y = iy + ry - ty;
+      if (y < 0 || y >= a.h) continue;
+      const float* row = G + (long long)y * a.gy_stride;
+      float v[KX];
+#pragma unroll
+      for (int tx = 0; tx < KX; ++tx) v[tx] = row[min(max(ix + rx - tx, 0), a.w - 1)];
+#pragma unroll
+      for (int tx = 0; tx < KX; ++tx) {
+        const int x = ix + rx - tx;
+        const float f = __builtin_fmaf(grad_weight(a, ty * KX + tx), v[tx], e);
+        e = (x >= 0 && x < a.w) ? f : e;
+      }
+    }
+  } else {
+    for (int ty = 0; ty < ky; ++ty) {
+      const int y = iy + ry - ty;
+      if (y < 0 || y >= a.h) continue;
+      for (int tx = 0; tx < kx; ++tx) {
+        const int x = ix + rx - tx;
+        if (x < 0 || x >= a.w) continue;
+        e = __builtin_fmaf(grad_weight(a, ty * kx + tx), G[(long long)y * a.gy_stride + x], e);
+      }
     }
   }
-  ring[b * rsize + j] = e;
+  ring[b * rsize + (long long)j] = e;
 }
 
-int launch_conv_ring(const ConvGradArgs& a, float* ring, hipStream_t s) {
+int launch_conv_ring(const ConvGradArgs& a, float* ring, float* wflip, hipStream_t s) {
   const long long rs = ring_size(a.w, a.h, a.kx / 2, a.ky / 2);
-  if (rs == 0) return HIPACC_OK;
+  if (rs == 0) return wflip ? launch_flip(a.wdev, wflip, a.kx * a.ky, s) : HIPACC_OK;
   dim3 grid((unsigned)((rs + 255) / 256), a.batch);
-  hipLaunchKernelGGL(conv_ring_kernel, grid, dim3(256), 0, s, a, ring, rs);
+  const bool small = rs + 256 < (1LL << 31);
+  if (small && a.kx == a.ky && (a.kx == 3 || a.kx == 5 || a.kx == 7 || a.kx == 9)) {
+    switch (a.kx) {
+      case 3: hipLaunchKernelGGL((conv_ring_kernel<3, 3, int>), grid, dim3(256), 0, s, a, ring, rs, wflip); break;
+      case 5: hipLaunchKernelGGL((conv_ring_kernel<5, 5, int>), grid, dim3(256), 0, s, a, ring, rs, wflip); break;
+      case 7: hipLaunchKernelGGL((conv_ring_kernel<7, 7, int>), grid, dim3(256), 0, s, a, ring, rs, wflip); break;
+      default: hipLaunchKernelGGL((conv_ring_kernel<9, 9, int>), grid, dim3(256), 0, s, a, ring, rs, wflip); break;
+    }
+  } else if (small) {
+    hipLaunchKernelGGL((conv_ring_kernel<0, 0, int>), grid, dim3(256), 0, s, a, ring, rs, wflip);
+  } else {
+    hipLaunchKernelGGL((conv_ring_kernel<0, 0, long long>), grid, dim3(256), 0, s, a, ring, rs,
+                       wflip);
+  }
   return hip_status(hipGetLastError(), "conv_ring_kernel launch");
 }
 
-// One thread per band pixel of one image.
+// One thread per band pixel of one image; with a reduction, kk more blocks
+// (of row 0 of the grid) sum the partials.  RX, RY > 0: the mask radii as
+// compile-time constants: the candidate positions of each axis (q, then
+// [-r, 0), then [n, n + r)) are unrolled and every ring load is issued
+// before the sum, in the order of the general loop below.
+template <int RX, int RY>
+__device__ __forceinline__ float fold_sum_fixed(int mode, int qx, int qy, int w, int h,
+                                                const float* R, bool& any) {
+  constexpr int NX = 2 * RX + 1, NY = 2 * RY + 1;
+  int cx[NX], cy[NY];
+  bool mx[NX], my[NY];
+  cx[0] = qx;
+  mx[0] = true;
+#pragma unroll
+  for (int k = 0; k < RX; ++k) {
+    cx[1 + k] = k - RX;
+    cx[1 + RX + k] = w + k;
+  }
+#pragma unroll
+  for (int k = 1; k < NX; ++k) mx[k] = bh_map(mode, cx[k], 0, w, w) == qx;
+  cy[0] = qy;
+  my[0] = true;
+#pragma unroll
+  for (int k = 0; k < RY; ++k) {
+    cy[1 + k] = k - RY;
+    cy[1 + RY + k] = h + k;
+  }
+#pragma unroll
+  for (int k = 1; k < NY; ++k) my[k] = bh_map(mode, cy[k], 0, h, h) == qy;
+  float v[NY][NX];
+  any = false;
+#pragma unroll
+  for (int jy = 0; jy < NY; ++jy)
+#pragma unroll
+    for (int jx = 0; jx < NX; ++jx) {
+      const bool on = my[jy] && mx[jx] && (jx | jy) != 0;
+      any |= on;
+      v[jy][jx] = on ? R[ring_index(w, h, RX, RY, cx[jx], cy[jy])] : 0.0f;
+    }
+  float acc = 0.0f;
+#pragma unroll
+  for (int jy = 0; jy < NY; ++jy)
+#pragma unroll
+    for (int jx = 0; jx < NX; ++jx)
+      if (my[jy] && mx[jx] && (jx | jy) != 0) acc += v[jy][jx];
+  return acc;
+}
+
+template <typename I, int RX, int RY>
 __global__ __launch_bounds__(256) void fold_add_kernel(float* gx, long long gx_plane,
                                                        int gx_stride, int w, int h, int rx,
                                                        int ry, int mode, const float* ring,
-                                                       long long rsize, long long bsize) {
-  const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
+                                                       long long rsize, long long bsize,
+                                                       int fold_blocks, const PartialSum sum) {
   const int b = blockIdx.y;
-  if (j >= bsize) return;
+  if ((int)blockIdx.x >= fold_blocks) {
+    if (b == 0) partial_sum_block(sum, (int)blockIdx.x - fold_blocks);
+    return;
+  }
+  const I j = (I)blockIdx.x * 256 + threadIdx.x;
+  if (j >= (I)bsize) return;
   int qx, qy;
-  band_pos(w, h, rx, ry, j, qx, qy);
+  band_pos<I>(w, h, rx, ry, j, qx, qy);
+  const float* R = ring + b * rsize;
+  if constexpr (RX > 0 && RY > 0) {
+    bool any;
+    const float acc = fold_sum_fixed<RX, RY>(mode, qx, qy, w, h, R, any);
+    if (!any) return;
+    float* dst = gx + b * gx_plane + (long long)qy * gx_stride + qx;
+    *dst = *dst + acc;
+    return;
+  }
   int px[2 * (HIPACC_MAX_MASK_DIM / 2) + 1], py[2 * (HIPACC_MAX_MASK_DIM / 2) + 1];
   const int nx = preimage(mode, qx, w, rx, px);
   const int ny = preimage(mode, qy, h, ry, py);
   if (nx == 1 && ny == 1) return;
-  const float* R = ring + b * rsize;
   float acc = 0.0f;
   for (int jy = 0; jy < ny; ++jy)
     for (int jx = 0; jx < nx; ++jx) {
@@ -92,14 +193,39 @@ __global__ __launch_bounds__(256) void fold_add_kernel(float* gx, long long gx_p
 }
 
 int launch_fold_add(float* gx, long long gx_plane, int gx_stride, int w, int h, int batch, int rx,
-                    int ry, int mode, const float* ring, hipStream_t s) {
-  if (mode == HIPACC_BOUNDARY_CONSTANT) return HIPACC_OK;
-  const long long bs = band_size(w, h, rx, ry), rs = ring_size(w, h, rx, ry);
-  if (bs == 0) return HIPACC_OK;
-  dim3 grid((unsigned)((bs + 255) / 256), batch);
-  hipLaunchKernelGGL(fold_add_kernel, grid, dim3(256), 0, s, gx, gx_plane, gx_stride, w, h, rx,
-                     ry, mode, ring, rs, bs);
+                    int ry, int mode, const float* ring, hipStream_t s, const PartialSum* sum) {
+  const long long bs = mode == HIPACC_BOUNDARY_CONSTANT ? 0 : band_size(w, h, rx, ry);
+  const long long rs = ring_size(w, h, rx, ry);
+  if (bs == 0) return sum ? launch_partial_sum(*sum, s) : HIPACC_OK;
+  const int fb = (int)((bs + 255) / 256);
+  PartialSum ps{};
+  if (sum) ps = *sum;
+  dim3 grid((unsigned)(fb + (sum ? sum->kk : 0)), batch);
+  const bool small = bs + 256 < (1LL << 31);
+  if (small && rx == ry && rx >= 1 && rx <= 4) {
+    switch (rx) {
+      case 1: hipLaunchKernelGGL((fold_add_kernel<int, 1, 1>), grid, dim3(256), 0, s, gx, gx_plane, gx_stride, w, h, rx, ry, mode, ring, rs, bs, fb, ps); break;
+      case 2: hipLaunchKernelGGL((fold_add_kernel<int, 2, 2>), grid, dim3(256), 0, s, gx, gx_plane, gx_stride, w, h, rx, ry, mode, ring, rs, bs, fb, ps); break;
+      case 3: hipLaunchKernelGGL((fold_add_kernel<int, 3, 3>), grid, dim3(256), 0, s, gx, gx_plane, gx_stride, w, h, rx, ry, mode, ring, rs, bs, fb, ps); break;
+      default: hipLaunchKernelGGL((fold_add_kernel<int, 4, 4>), grid, dim3(256), 0, s, gx, gx_plane, gx_stride, w, h, rx, ry, mode, ring, rs, bs, fb, ps); break;
+    }
+  } else if (small) {
+    hipLaunchKernelGGL((fold_add_kernel<int, 0, 0>), grid, dim3(256), 0, s, gx, gx_plane,
+                       gx_stride, w, h, rx, ry, mode, ring, rs, bs, fb, ps);
+  } else {
+    hipLaunchKernelGGL((fold_add_kernel<long long, 0, 0>), grid, dim3(256), 0, s, gx, gx_plane,
+                       gx_stride, w, h, rx, ry, mode, ring, rs, bs, fb, ps);
+  }
   return hip_status(hipGetLastError(), "fold_add_kernel launch");
+}
+
+__global__ __launch_bounds__(256) void partial_sum_kernel(const PartialSum p) {
+  partial_sum_block(p, blockIdx.x);
+}
+
+int launch_partial_sum(const PartialSum& p, hipStream_t s) {
+  hipLaunchKernelGGL(partial_sum_kernel, dim3(p.kk), dim3(256), 0, s, p);
+  return hip_status(hipGetLastError(), "partial_sum_kernel launch");
 }
 
 // dw over 64 x 32 pixel tiles of square K = 2R+1 masks.  Each lane owns 4
@@ -456,20 +582,7 @@ __global__ __launch_bounds__(256) void conv_dw_generic(const ConvGradArgs a) {
     a.partials[(long long)blockIdx.x * kk + t] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
-// dw[t] = SUM_b partials[b][t] in a fixed order.
-__global__ __launch_bounds__(256) void conv_dw_reduce(const float* partials, int nblocks, int kk,
-                                                      float* dw) {
-  __shared__ float red[4];
-  const int t = blockIdx.x;
-  float s = 0.0f;
-  for (int b = threadIdx.x; b < nblocks; b += 256) s += partials[(long long)b * kk + t];
-  s = wave_sum(s);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) dw[t] = ((red[0] + red[1]) + red[2]) + red[3];
-}
-
-int launch_conv_dw(const ConvGradArgs& a0, float* dw, hipStream_t s) {
+int launch_conv_dw(const ConvGradArgs& a0, float* dw, hipStream_t s, PartialSum* sum) {
   ConvGradArgs a = a0;
   const int kk = a.kx * a.ky;
   int nblocks;
@@ -512,10 +625,12 @@ int launch_conv_dw(const ConvGradArgs& a0, float* dw, hipStream_t s) {
     nblocks = (int)(nc < 256 ? nc : 256);
     hipLaunchKernelGGL(conv_dw_generic, dim3(nblocks, kk), dim3(256), 0, s, a);
   }
-  int st = hip_status(hipGetLastError(), "conv_dw launch");
-  if (st) return st;
-  hipLaunchKernelGGL(conv_dw_reduce, dim3(kk), dim3(256), 0, s, a.partials, nblocks, kk, dw);
-  return hip_status(hipGetLastError(), "conv_dw_reduce launch");
+  sum->partials = a.partials;
+  sum->nblocks = nblocks;
+  sum->kk = kk;
+  sum->scale = 1.0f;
+  sum->out = dw;
+  return hip_status(hipGetLastError(), "conv_dw launch");
 }
 
 }  // namespace hh

@@ -42,20 +42,23 @@ HH_HD long long ring_index(int w, int h, int rx, int ry, int ix, int iy) {
   if (ix < 0) return base + (long long)iy * rx + (ix + rx);
   return base + (long long)h * rx + (long long)iy * rx + (ix - w);
 }
-HH_HD void ring_pos(int w, int h, int rx, int ry, long long j, int& ix, int& iy) {
-  const long long ew = w + 2 * rx;
-  if (j < 2LL * ry * ew) {
-    const long long r = j / ew;
+// I: the index type (32-bit where the ring of one image has fewer than
+// 2^31 positions).
+template <typename I>
+HH_HD void ring_pos(int w, int h, int rx, int ry, I j, int& ix, int& iy) {
+  const I ew = w + 2 * rx;
+  if (j < (I)2 * ry * ew) {
+    const I r = j / ew;
     ix = (int)(j - r * ew) - rx;
     iy = r < ry ? (int)r - ry : h + (int)(r - ry);
     return;
   }
-  j -= 2LL * ry * ew;
-  const long long side = (long long)h * rx;
+  j -= (I)2 * ry * ew;
+  const I side = (I)h * rx;
   const bool right = j >= side;
   if (right) j -= side;
   iy = (int)(j / rx);
-  const int c = (int)(j - (long long)iy * rx);
+  const int c = (int)(j - (I)iy * rx);
   ix = right ? w + c : c - rx;
 }
 
@@ -72,13 +75,17 @@ HH_HD long long band_size(int w, int h, int rx, int ry) {
   const int lcols = side, rcols = w - (w - rx > side ? w - rx : side);
   return (long long)(top + (h - bot0)) * w + (long long)(bot0 - top) * (lcols + rcols);
 }
-HH_HD void band_pos(int w, int h, int rx, int ry, long long j, int& x, int& y) {
+// I: the index type, 32-bit where the band of one image has fewer than 2^31
+// pixels (the divisions then take a few instructions instead of a 64-bit
+// division sequence).
+template <typename I>
+HH_HD void band_pos(int w, int h, int rx, int ry, I j, int& x, int& y) {
   int top, bot0, side;
   band_rows(w, h, rx, ry, top, bot0, side);
-  const long long full = (long long)(top + (h - bot0)) * w;
+  const I full = (I)(top + (h - bot0)) * w;
   if (j < full) {
     const int r = (int)(j / w);
-    x = (int)(j - (long long)r * w);
+    x = (int)(j - (I)r * w);
     y = r < top ? r : bot0 + (r - top);
     return;
   }
@@ -86,14 +93,37 @@ HH_HD void band_pos(int w, int h, int rx, int ry, long long j, int& x, int& y) {
   const int rstart = w - rx > side ? w - rx : side;
   const int lcols = side, rcols = w - rstart, per = lcols + rcols;
   y = top + (int)(j / per);
-  const int c = (int)(j - (long long)(y - top) * per);
+  const int c = (int)(j - (I)(y - top) * per);
   x = c < lcols ? c : rstart + (c - lcols);
 }
 
+// out[t] = scale * SUM_b partials[b * kk + t], t < kk, summed in a fixed
+// order (deterministic run to run).
+struct PartialSum {
+  const float* partials;
+  int nblocks, kk;
+  float scale;
+  float* out;
+};
+// Block t of a reduction launch (256 threads) computes out[t].
+__device__ __forceinline__ void partial_sum_block(const PartialSum& p, int t) {
+  __shared__ float red[4];
+  float s = 0.0f;
+  for (int b = threadIdx.x; b < p.nblocks; b += 256) s += p.partials[(long long)b * p.kk + t];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) p.out[t] = (((red[0] + red[1]) + red[2]) + red[3]) * p.scale;
+}
+int launch_partial_sum(const PartialSum& p, hipStream_t s);
+
 // Adds, for every band pixel q, the ring values of the extended positions
-// (other than q) its boundary mode maps onto q, in a fixed order.
+// (other than q) its boundary mode maps onto q, in a fixed order.  With
+// `sum` (not NULL), the same launch also performs that reduction (the
+// partials are complete when the fold starts).
 int launch_fold_add(float* gx, long long gx_plane, int gx_stride, int w, int h, int batch, int rx,
-                    int ry, int mode, const float* ring, hipStream_t s);
+                    int ry, int mode, const float* ring, hipStream_t s,
+                    const PartialSum* sum = nullptr);
 
 // Adjoint of out(p) = SUM_t w[t] * X_bh(p + t - r) over whole images (the
 // accessor window is the image, the iteration space has its size).
@@ -117,10 +147,12 @@ struct ConvGradArgs {
 
 // E(i) = SUM_t w[t] * dY0(i + r - t) (dY0: dY inside the image, 0 outside)
 // at every ring position i, into ring[b * ring_size + ring_index(i)].
-int launch_conv_ring(const ConvGradArgs& a, float* ring, hipStream_t s);
-// dw (device, kx*ky floats) via per-block partials in a.partials and a
-// fixed-order reduction.
-int launch_conv_dw(const ConvGradArgs& a, float* dw, hipStream_t s);
+// With wflip (not NULL) the same launch writes the flipped device mask
+// wflip[t] = wdev[kx*ky-1-t] for the dX pass.
+int launch_conv_ring(const ConvGradArgs& a, float* ring, float* wflip, hipStream_t s);
+// Per-block partial sums of dw into a.partials (the reduction, `sum`, is
+// left to the caller: launch_partial_sum or a fold launch).
+int launch_conv_dw(const ConvGradArgs& a, float* dw, hipStream_t s, PartialSum* sum);
 // out[t] = w[n-1-t] (the flipped mask of the dX correlation).
 int launch_flip(const float* w, float* out, int n, hipStream_t s);
 
@@ -143,6 +175,8 @@ struct BilatGradArgs {
   float lsp[kMaxTaps]; // log2 of the spatial weights
   int sym;             // sp is point-symmetric
   float* partials;     // [blocks] partial sums of dL/dc
+  float* ring;         // neighbour terms of the ring positions (NULL: none)
+  long long rsize;     // ring positions per image
   int vec_ok;
   int tiles_x, tiles_y, ntiles;
 };
