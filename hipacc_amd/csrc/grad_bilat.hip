@@ -10,12 +10,45 @@
 // tap -o, and exp(-c d^2) is the same for the pair, so one exponential per
 // (q, o) serves both terms.  Pixels within r of an edge also receive the
 // neighbour terms of the halo positions the boundary mode folds onto them:
-// those are evaluated once per halo (ring) position, by the tile whose
-// pixels the position clamps to (its staged halo holds every pixel that
-// reads the position), and added afterwards.
+// those are evaluated once per halo (ring) position and added afterwards.
 #include "hh_grad.hpp"
 
 namespace hh {
+
+// Neighbour terms landing on ring position i (outside the image), read by
+// the taps of in-image pixels p = i - o_t; the value read there is X(q),
+// q = bh(i).  One thread per ring position of one image.
+__global__ __launch_bounds__(256) void bilat_ring_kernel(const BilatGradArgs a) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  const int b = blockIdx.y;
+  if (j >= (int)a.rsize) return;
+  const int r = a.ksize / 2;
+  int ix, iy;
+  ring_pos<int>(a.w, a.h, r, r, j, ix, iy);
+  const int qx = bh_map(a.boundary, ix, 0, a.w, a.w), qy = bh_map(a.boundary, iy, 0, a.h, a.h);
+  const float* X = a.x + b * a.x_plane;
+  const float* Y = a.y + b * a.y_plane;
+  const float* D = a.d + b * a.d_plane;
+  const float* G = a.gy + b * a.gy_plane;
+  const float xq = X[(long long)qy * a.x_stride + qx];
+  float acc = 0.0f;
+  for (int ty = 0; ty < a.ksize; ++ty) {
+    const int y = iy - (ty - r);
+    if (y < 0 || y >= a.h) continue;
+    for (int tx = 0; tx < a.ksize; ++tx) {
+      const int x = ix - (tx - r);
+      if (x < 0 || x >= a.w) continue;
+      const float xp = X[(long long)y * a.x_stride + x];
+      const float yp = Y[(long long)y * a.y_stride + x];
+      const float ap = G[(long long)y * a.gy_stride + x] / D[(long long)y * a.d_stride + x];
+      const float dd = xp - xq;
+      const float sv = __builtin_amdgcn_exp2f(a.k2 * (dd * dd)) * a.sp[ty * a.ksize + tx];
+      const float w = ap * sv;
+      acc += w + 2.0f * a.c * (w * dd) * (xq - yp);
+    }
+  }
+  a.ring[b * a.rsize + j] = acc;
+}
 
 // 64 x 16 pixel tiles, 4 adjacent pixels per lane.  LDS holds, with a halo of
 // r rows and RH columns: X_bh (CONSTANT: the constant), a = dY/D (0 outside
@@ -190,54 +223,6 @@ __global__ __launch_bounds__(256) void bilat_grad_kernel(const BilatGradArgs a) 
             if (x0 + tc + j < a.w) GX[x0 + tc + j] = v[j];
         }
       }
-      // Ring positions i (outside the image, within R of it) that clamp to
-      // a pixel of this tile: the neighbour terms of the in-image pixels p
-      // whose taps read i, where the value read is X(bh(i)).  Every such p
-      // lies within R of i, inside the staged rows and columns (only
-      // tiles at an image edge have ring positions, and they stage through
-      // the boundary map: xs holds X(bh(i)), as and ys hold 0 outside).
-      if (a.ring && !interior) {
-        const int x1 = min(x0 + TW, a.w), y1 = min(y0 + TH, a.h);
-        const int ex0 = x0 == 0 ? -R : x0, ex1 = x1 == a.w ? a.w + R : x1;
-        const int ey0 = y0 == 0 ? -R : y0, ey1 = y1 == a.h ? a.h + R : y1;
-        // enumerated densely (whole rows above and below the image, then
-        // the columns left and right of it), so that the lanes of a wave
-        // take consecutive positions
-        const int ix0 = max(ex0, 0), ix1 = min(ex1, a.w), iy0 = max(ey0, 0), iy1 = min(ey1, a.h);
-        const int ew = ex1 - ex0, tr0 = iy0 - ey0, br = ey1 - iy1;
-        const int lc = ix0 - ex0, sc = lc + (ex1 - ix1);
-        const int nfull = (tr0 + br) * ew, n = nfull + (iy1 - iy0) * sc;
-        float* ring = a.ring + b * a.rsize;
-        for (int i = tid; i < n; i += 256) {
-          int ix, iy;
-          if (i < nfull) {
-            const int ri = i / ew;
-            iy = ri < tr0 ? ey0 + ri : iy1 + (ri - tr0);
-            ix = ex0 + (i - ri * ew);
-          } else {
-            const int k = i - nfull, ri = k / sc, c = k - ri * sc;
-            iy = iy0 + ri;
-            ix = c < lc ? ex0 + c : ix1 + (c - lc);
-          }
-          const float xq = xs[(iy - y0 + R) * LW + (ix - x0 + RH)];
-          float acc = 0.0f;
-          for (int ty = 0; ty < K; ++ty) {
-            const int y = iy - (ty - R);
-            if (y < 0 || y >= a.h) continue;
-            for (int tx = 0; tx < K; ++tx) {
-              const int x = ix - (tx - R);
-              if (x < 0 || x >= a.w) continue;
-              const int o = (y - y0 + R) * LW + (x - x0 + RH);
-              const float xp = xs[o], yp = ys[o], ap = as[o];
-              const float dd = xp - xq;
-              const float sv = __builtin_amdgcn_exp2f(a.k2 * (dd * dd)) * a.sp[ty * K + tx];
-              const float w = ap * sv;
-              acc += w + 2.0f * a.c * (w * dd) * (xq - yp);
-            }
-          }
-          ring[ring_index(a.w, a.h, R, R, ix, iy)] = acc;
-        }
-      }
     }
   }
 
@@ -245,6 +230,12 @@ __global__ __launch_bounds__(256) void bilat_grad_kernel(const BilatGradArgs a) 
   if ((tid & 63) == 0) red[tid >> 6] = dc;
   __syncthreads();
   if (tid == 0) a.partials[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+static int launch_bilat_ring(const BilatGradArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(bilat_ring_kernel, dim3((unsigned)((a.rsize + 255) / 256), a.batch),
+                     dim3(256), 0, s, a);
+  return hip_status(hipGetLastError(), "bilat_ring_kernel launch");
 }
 
 template <int R>
@@ -269,10 +260,13 @@ int launch_bilateral_grad(const BilatGradArgs& a0, float* dsigma, float sigma_r,
   a.ring = a.gx && a.boundary != HIPACC_BOUNDARY_CONSTANT ? ring : nullptr;
   a.rsize = ring_size(a.w, a.h, a.ksize / 2, a.ksize / 2);
   if (a.rsize == 0) a.ring = nullptr;
+  if (a.ring && a.rsize + 256 >= (1LL << 31))
+    return set_error(HIPACC_ERR_UNSUPPORTED, "bilateral_grad: image too large for the halo ring");
   a.tiles_x = ceil_div(a.w, 64);
   a.tiles_y = ceil_div(a.h, 16);
   a.ntiles = a.tiles_x * a.tiles_y * a.batch;
   const int nblocks = a.ntiles < kGradBlocks ? a.ntiles : kGradBlocks;
+  if (a.ksize > 15) return set_error(HIPACC_ERR_UNSUPPORTED, "bilateral_grad: ksize %d", a.ksize);
   int st;
   switch (a.ksize) {
     case 1: st = launch_bg<0>(a, nblocks, s); break;
@@ -285,6 +279,7 @@ int launch_bilateral_grad(const BilatGradArgs& a0, float* dsigma, float sigma_r,
     case 15: st = launch_bg<7>(a, nblocks, s); break;
     default: return set_error(HIPACC_ERR_UNSUPPORTED, "bilateral_grad: ksize %d", a.ksize);
   }
+  if (st == HIPACC_OK && a.ring) st = launch_bilat_ring(a, s);
   // dsigma = -1/sigma^3 * SUM_b partials[b]: with the fold when there is one
   PartialSum ps{a.partials, nblocks, 1, (float)(-1.0 / ((double)sigma_r * sigma_r * sigma_r)),
                 dsigma};
