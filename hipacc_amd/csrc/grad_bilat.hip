@@ -15,39 +15,53 @@
 
 namespace hh {
 
-// Neighbour terms landing on ring position i (outside the image), read by
-// the taps of in-image pixels p = i - o_t; the value read there is X(q),
-// q = bh(i).  One thread per ring position of one image.
+// Neighbour terms landing on ring position i (outside the image, within R
+// of it), read by the taps of the in-image pixels p = i - o_t; the value
+// read there is X(q), q = bh(i).  K threads per position, one per tap row:
+// each sums its row's taps inside the image, and the position's first
+// thread adds the K row sums in row order.
+template <int R>
 __global__ __launch_bounds__(256) void bilat_ring_kernel(const BilatGradArgs a) {
-  const int j = blockIdx.x * 256 + threadIdx.x;
+  constexpr int K = 2 * R + 1, P = 256 / K;   // positions per block
+  __shared__ float rows[P * K];
+  const int t = threadIdx.x, pl = t / K, ty = t - pl * K;
+  const int j = blockIdx.x * P + pl;
   const int b = blockIdx.y;
-  if (j >= (int)a.rsize) return;
-  const int r = a.ksize / 2;
-  int ix, iy;
-  ring_pos<int>(a.w, a.h, r, r, j, ix, iy);
-  const int qx = bh_map(a.boundary, ix, 0, a.w, a.w), qy = bh_map(a.boundary, iy, 0, a.h, a.h);
-  const float* X = a.x + b * a.x_plane;
-  const float* Y = a.y + b * a.y_plane;
-  const float* D = a.d + b * a.d_plane;
-  const float* G = a.gy + b * a.gy_plane;
-  const float xq = X[(long long)qy * a.x_stride + qx];
+  const bool live = pl < P && j < (int)a.rsize;
   float acc = 0.0f;
-  for (int ty = 0; ty < a.ksize; ++ty) {
-    const int y = iy - (ty - r);
-    if (y < 0 || y >= a.h) continue;
-    for (int tx = 0; tx < a.ksize; ++tx) {
-      const int x = ix - (tx - r);
-      if (x < 0 || x >= a.w) continue;
-      const float xp = X[(long long)y * a.x_stride + x];
-      const float yp = Y[(long long)y * a.y_stride + x];
-      const float ap = G[(long long)y * a.gy_stride + x] / D[(long long)y * a.d_stride + x];
-      const float dd = xp - xq;
-      const float sv = __builtin_amdgcn_exp2f(a.k2 * (dd * dd)) * a.sp[ty * a.ksize + tx];
-      const float w = ap * sv;
-      acc += w + 2.0f * a.c * (w * dd) * (xq - yp);
+  int ix = 0, iy = 0;
+  if (live) {
+    ring_pos<int>(a.w, a.h, R, R, j, ix, iy);
+    const int y = iy - (ty - R);
+    if (y >= 0 && y < a.h) {
+      const int qx = bh_map(a.boundary, ix, 0, a.w, a.w);
+      const int qy = bh_map(a.boundary, iy, 0, a.h, a.h);
+      const float* X = a.x + b * a.x_plane;
+      const float* Y = a.y + b * a.y_plane;
+      const float* D = a.d + b * a.d_plane;
+      const float* G = a.gy + b * a.gy_plane;
+      const float xq = X[(long long)qy * a.x_stride + qx];
+      for (int tx = 0; tx < K; ++tx) {
+        const int x = ix - (tx - R);
+        if (x < 0 || x >= a.w) continue;
+        const float xp = X[(long long)y * a.x_stride + x];
+        const float yp = Y[(long long)y * a.y_stride + x];
+        const float ap = G[(long long)y * a.gy_stride + x] / D[(long long)y * a.d_stride + x];
+        const float dd = xp - xq;
+        const float sv = __builtin_amdgcn_exp2f(a.k2 * (dd * dd)) * a.sp[ty * K + tx];
+        const float w = ap * sv;
+        acc += w + 2.0f * a.c * (w * dd) * (xq - yp);
+      }
     }
   }
-  a.ring[b * a.rsize + j] = acc;
+  if (pl < P) rows[t] = acc;
+  __syncthreads();
+  if (live && ty == 0) {
+    float sum = rows[pl * K];
+#pragma unroll
+    for (int r = 1; r < K; ++r) sum += rows[pl * K + r];
+    a.ring[b * a.rsize + j] = sum;
+  }
 }
 
 // 64 x 16 pixel tiles, 4 adjacent pixels per lane.  LDS holds, with a halo of
@@ -232,8 +246,10 @@ __global__ __launch_bounds__(256) void bilat_grad_kernel(const BilatGradArgs a) 
   if (tid == 0) a.partials[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
+template <int R>
 static int launch_bilat_ring(const BilatGradArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(bilat_ring_kernel, dim3((unsigned)((a.rsize + 255) / 256), a.batch),
+  constexpr int P = 256 / (2 * R + 1);
+  hipLaunchKernelGGL(bilat_ring_kernel<R>, dim3((unsigned)((a.rsize + P - 1) / P), a.batch),
                      dim3(256), 0, s, a);
   return hip_status(hipGetLastError(), "bilat_ring_kernel launch");
 }
@@ -279,7 +295,17 @@ int launch_bilateral_grad(const BilatGradArgs& a0, float* dsigma, float sigma_r,
     case 15: st = launch_bg<7>(a, nblocks, s); break;
     default: return set_error(HIPACC_ERR_UNSUPPORTED, "bilateral_grad: ksize %d", a.ksize);
   }
-  if (st == HIPACC_OK && a.ring) st = launch_bilat_ring(a, s);
+  if (st == HIPACC_OK && a.ring) {
+    switch (a.ksize) {
+      case 3: st = launch_bilat_ring<1>(a, s); break;
+      case 5: st = launch_bilat_ring<2>(a, s); break;
+      case 7: st = launch_bilat_ring<3>(a, s); break;
+      case 9: st = launch_bilat_ring<4>(a, s); break;
+      case 11: st = launch_bilat_ring<5>(a, s); break;
+      case 13: st = launch_bilat_ring<6>(a, s); break;
+      default: st = launch_bilat_ring<7>(a, s); break;
+    }
+  }
   // dsigma = -1/sigma^3 * SUM_b partials[b]: with the fold when there is one
   PartialSum ps{a.partials, nblocks, 1, (float)(-1.0 / ((double)sigma_r * sigma_r * sigma_r)),
                 dsigma};
