@@ -108,7 +108,7 @@ def test_conv_adjoint_identity_full_size(gpu, mode):
 
 
 @pytest.mark.parametrize("mode", MODES)
-@pytest.mark.parametrize("k", [3, 5, 7])
+@pytest.mark.parametrize("k", [3, 5, 7, 9, 15])
 @pytest.mark.parametrize("shape", [(2, 37, 29), (1, 70, 131)])
 def test_bilateral_grad_matches_oracle(gpu, mode, k, shape):
     print(f"bilateral_grad mode={mode} k={k} shape={shape}", flush=True)
